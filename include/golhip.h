@@ -1,0 +1,156 @@
+/*
+ * golhip — C ABI of the MI355X Game-of-Life engine (B3/S23, cells outside
+ * the grid dead).
+ *
+ * This is the boundary a host program binds to. It replaces the CUDA side of
+ * the reference's algorithm plug-ins (the device allocation, copies and
+ * per-generation launch loop inside initialize_data_structures / run /
+ * finalize_data_structures of an infrastructure::Algorithm<2, cell_t>,
+ * src/infrastructure/algorithm.hpp:37-69), so that a plug-in class compiled
+ * with a plain C++ compiler can drive the GPU through plain pointers and sizes.
+ *
+ * Word layout ("rows"), as the reference's FullyPackedRows policy
+ * (src/algorithms/_shared/bitwise/bit_modes.hpp:127-163,
+ *  src/algorithms/_shared/bitwise/general_bit_grid.hpp:217-246):
+ *   word (wx, y) is at index y * (X / W) + wx, bit k (value 1 << k) is cell
+ *   (W * wx + k, y), for W = word_bits = 32 or 64. On a little-endian host the
+ *   two layouts are the same bytes, so W only fixes the width granularity and
+ *   the unit of the word counts below.
+ *
+ * Conventions:
+ *   - every call returns GOL_OK (0) or an error code; gol_last_error() gives
+ *     the calling thread's last message;
+ *   - the caller owns host buffers, the library owns device buffers;
+ *   - one handle is used by one host thread at a time;
+ *   - gol_step only enqueues work on the handle's stream; gol_download and
+ *     gol_sync wait for it.
+ */
+#ifndef GOLHIP_H
+#define GOLHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The C++ plug-in wrapper maps them onto the exceptions the
+ * reference throws: */
+enum gol_status {
+    GOL_OK = 0,
+    GOL_ERR_INVALID_ARGUMENT = 1, /* std::invalid_argument (general_bit_grid.hpp:208-215) */
+    GOL_ERR_RUNTIME = 2,          /* std::runtime_error (temporal_rowed.cu:190-206) */
+    GOL_ERR_HIP = 3,              /* CudaError raised by CUCH (_shared/cuda-helpers/cuch.hpp:42-52) */
+    GOL_ERR_OUT_OF_MEMORY = 4,    /* cudaMalloc failure inside CUCH */
+    GOL_ERR_NOT_FOUND = 5,        /* unknown job / handle */
+    GOL_ERR_BUSY = 6              /* resource temporarily full (admission) */
+};
+
+/* Which kernel family advances the grid. */
+enum gol_kernel {
+    GOL_KERNEL_AUTO = 0,     /* temporal tiles, shape and depth from the built-in table */
+    GOL_KERNEL_PLAIN = 1,    /* one generation per launch
+                                (reference: cuda-naive-bitwise/cuda_naive_bitwise_kernel.cu:32-79) */
+    GOL_KERNEL_TEMPORAL = 2, /* T generations per launch on LDS-exchanged register tiles
+                                (reference: cuda-temporal-rowed/temporal_rowed.cu:33-229) */
+    GOL_KERNEL_ACTIVE = 3    /* change tracking over a device-side active-tile list
+                                (reference: cuda-naive-local-one-cell/cuda_local_one_cell.cu:53-242) */
+};
+
+typedef struct gol_config {
+    uint64_t x_cells;        /* grid width in cells, a multiple of word_bits */
+    uint64_t y_cells;        /* grid height in cells (rows) */
+    uint32_t word_bits;      /* 32 or 64 */
+    uint32_t kernel;         /* enum gol_kernel */
+    uint32_t temporal_steps; /* generations per temporal launch; 0 = chosen by the library
+                                (reference: ExperimentParams::temporal_steps,
+                                 src/infrastructure/experiment_params.hpp:62) */
+    uint32_t lane_dwords;    /* dwords per lane 1/2/4/8; 0 = chosen by the library */
+    uint32_t band_rows;      /* rows per wavefront band; 0 = chosen by the library */
+    uint32_t waves;          /* wavefronts per workgroup; 0 = chosen by the library */
+    int32_t device;          /* HIP device ordinal; -1 = current device */
+    uint32_t reserved;
+} gol_config;
+
+typedef struct gol_info {
+    uint32_t kernel;          /* resolved enum gol_kernel */
+    uint32_t lane_dwords;     /* resolved tile shape */
+    uint32_t band_rows;
+    uint32_t waves;
+    uint32_t temporal_steps;  /* generations per launch */
+    uint32_t tiles;           /* workgroups per launch */
+    uint64_t launches;        /* kernel launches issued so far */
+    uint64_t generations;     /* generations enqueued so far (actually_performed_iterations) */
+    uint64_t last_active_tiles; /* GOL_KERNEL_ACTIVE: tiles computed in the last generation */
+    uint64_t device_bytes;    /* device memory owned by the handle */
+} gol_info;
+
+typedef struct gol_handle gol_handle;
+
+/* Allocate device state for one grid (reference: cudaMalloc in
+ * initialize_data_structures, e.g. cuda-temporal-rowed/temporal_rowed.hpp:34-44). */
+int gol_create(const gol_config* config, gol_handle** out);
+
+/* Copy n_words host words (word_bits each) in as the current state
+ * (reference: cudaMemcpy H2D in initialize_data_structures). */
+int gol_upload(gol_handle* h, const void* words, size_t n_words);
+
+/* Copy n_words words already in device memory (e.g. another handle's
+ * gol_device_state) in as the current state, ordered on the handle's stream. */
+int gol_upload_device(gol_handle* h, const void* device_words, size_t n_words);
+
+/* Enqueue `generations` generations (0 = identity)
+ * (reference: Algorithm::run(iterations), src/infrastructure/algorithm.hpp:49). */
+int gol_step(gol_handle* h, uint64_t generations);
+
+/* Same as gol_step, then wait and return the GPU time of the launches in ms,
+ * measured with HIP events on the handle's stream. */
+int gol_step_timed(gol_handle* h, uint64_t generations, double* gpu_ms);
+
+/* Wait for the handle's stream, then copy the current state out
+ * (reference: cudaMemcpy D2H in finalize_data_structures,
+ *  e.g. cuda-temporal-rowed/temporal_rowed.hpp:53-62). */
+int gol_download(gol_handle* h, void* words, size_t n_words);
+
+/* Wait for all work enqueued on the handle
+ * (reference: the cudaDeviceSynchronize of Timer::measure_ms, src/infrastructure/timer.hpp:15-26). */
+int gol_sync(gol_handle* h);
+
+/* Device pointer to the current state (valid until the next gol_step) and
+ * the hipStream_t the handle launches on. */
+int gol_device_state(gol_handle* h, void** device_words);
+int gol_get_stream(gol_handle* h, void** stream);
+
+int gol_get_info(gol_handle* h, gol_info* info);
+
+/* Release the device state (reference: cudaFree in finalize_data_structures). */
+void gol_destroy(gol_handle* h);
+
+/* Host-side input generators (no device work), writing packed row words:
+ *  gol_fill_random: the reference's RandomOnesZerosDataLoader
+ *    (src/infrastructure/data_loader.hpp:57-75): std::mt19937(seed),
+ *    uniform_int_distribution<int>(0,1), one draw per cell in y*X+x order;
+ *  gol_fill_bernoulli: Bernoulli(p) cells, u = (mt19937_64() >> 11) * 2^-53 < p
+ *    (no reference counterpart: the reference only draws p = 0.5);
+ *  gol_fill_patches: empty grid with n_patches squares of side `patch`, corner
+ *    (rng() % (X-patch+1), rng() % (Y-patch+1)) then Bernoulli(p) cells from the
+ *    same mt19937_64(seed) stream, later patches overwriting earlier ones
+ *    (no reference counterpart: the mostly-stable change-tracking workload). */
+int gol_fill_random(void* words, uint64_t x_cells, uint64_t y_cells, uint32_t word_bits, uint64_t seed);
+int gol_fill_bernoulli(void* words, uint64_t x_cells, uint64_t y_cells, uint32_t word_bits, uint64_t seed,
+                       double p);
+int gol_fill_patches(void* words, uint64_t x_cells, uint64_t y_cells, uint32_t word_bits, uint64_t seed,
+                     uint32_t n_patches, uint32_t patch, double p);
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char* gol_last_error(void);
+
+/* Library version string. */
+const char* gol_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GOLHIP_H */

@@ -1,0 +1,335 @@
+// C ABI over the HIP kernels: one handle = one grid's device state, its
+// ping-pong buffers and its stream. Declarations and the reference
+// interfaces each call replaces are in include/golhip.h.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/golhip.h"
+#include "gol_capi_internal.hpp"
+#include "gol_kernels.hpp"
+
+namespace golhip {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+int fail_hip(hipError_t e, const char* what) {
+    return fail(e == hipErrorOutOfMemory ? GOL_ERR_OUT_OF_MEMORY : GOL_ERR_HIP, "%s: %s (%d)", what,
+                hipGetErrorString(e), static_cast<int>(e));
+}
+
+void clear_error() { g_last_error.clear(); }
+
+const char* last_error() { return g_last_error.c_str(); }
+
+// Default tile shape by row width: the narrowest lane width whose wavefront
+// spans the row, so one tile covers the full width when it can (no x halo).
+TileShape default_shape(int64_t x_dw, int64_t y_rows) {
+    if (x_dw <= 64) return (y_rows <= 512 && y_rows > 128) ? TileShape{1, 32, 16} : TileShape{1, 16, 8};
+    if (x_dw <= 128) return TileShape{2, 16, 8};
+    if (x_dw <= 256) return TileShape{4, 8, 8};
+    return TileShape{8, 8, 8};
+}
+
+bool shape_compiled(const TileShape& s) {
+    for (int i = 0; i < kNumTileShapes; ++i) {
+        if (kTileShapes[i].lane_dwords == s.lane_dwords && kTileShapes[i].band_rows == s.band_rows &&
+            kTileShapes[i].waves == s.waves)
+            return true;
+    }
+    return false;
+}
+
+}  // namespace golhip
+
+using namespace golhip;
+
+struct gol_handle {
+    gol_config cfg;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    uint32_t* buf[2] = {nullptr, nullptr};
+    int cur = 0;
+    int64_t x_dw = 0, y_rows = 0;
+    size_t n_dwords = 0;
+    uint32_t kernel = GOL_KERNEL_TEMPORAL;
+    uint32_t plain_dw = 1;
+    TileShape shape{1, 16, 8};
+    int steps = 8;        // generations per temporal launch
+    bool one_tile = false;  // one tile spans the grid: no halo, any number of generations per launch
+    TileGeometry geom{};
+    uint64_t launches = 0;
+    uint64_t generations = 0;
+};
+
+namespace {
+
+int validate_config(const gol_config* c) {
+    if (!c) return fail(GOL_ERR_INVALID_ARGUMENT, "config is null");
+    if (c->word_bits != 32 && c->word_bits != 64)
+        return fail(GOL_ERR_INVALID_ARGUMENT, "word_bits must be 32 or 64, got %u", c->word_bits);
+    if (c->x_cells == 0 || c->y_cells == 0)
+        return fail(GOL_ERR_INVALID_ARGUMENT, "grid dimensions must be positive");
+    if (c->x_cells % c->word_bits != 0)
+        return fail(GOL_ERR_INVALID_ARGUMENT, "Grid dimensions X must be a multiple of %u", c->word_bits);
+    if (c->x_cells / 32 > (1u << 26) || c->y_cells > (1u << 30) || (c->x_cells / 32) * c->y_cells > (1ull << 34))
+        return fail(GOL_ERR_INVALID_ARGUMENT, "grid too large");
+    if (c->kernel > GOL_KERNEL_ACTIVE) return fail(GOL_ERR_INVALID_ARGUMENT, "unknown kernel %u", c->kernel);
+    if (c->lane_dwords != 0 && c->lane_dwords != 1 && c->lane_dwords != 2 && c->lane_dwords != 4 &&
+        c->lane_dwords != 8)
+        return fail(GOL_ERR_INVALID_ARGUMENT, "lane_dwords must be 0, 1, 2, 4 or 8");
+    return GOL_OK;
+}
+
+// Resolve kernel, tile shape and generations per launch for a handle.
+int plan(gol_handle* h) {
+    const gol_config& c = h->cfg;
+    h->kernel = c.kernel == GOL_KERNEL_AUTO ? GOL_KERNEL_TEMPORAL : c.kernel;
+    if (h->kernel == GOL_KERNEL_ACTIVE)
+        return fail(GOL_ERR_INVALID_ARGUMENT, "GOL_KERNEL_ACTIVE is not available in this build");
+    if (h->kernel == GOL_KERNEL_PLAIN) {
+        uint32_t dw = c.lane_dwords;
+        if (dw == 0) dw = h->x_dw <= 64 ? 1 : (h->x_dw <= 128 ? 2 : 4);
+        h->plain_dw = dw;
+        return GOL_OK;
+    }
+    TileShape s = default_shape(h->x_dw, h->y_rows);
+    if (c.lane_dwords) s.lane_dwords = static_cast<int>(c.lane_dwords);
+    if (c.band_rows) s.band_rows = static_cast<int>(c.band_rows);
+    if (c.waves) s.waves = static_cast<int>(c.waves);
+    if (!shape_compiled(s))
+        return fail(GOL_ERR_RUNTIME, "tile shape (lane_dwords %d, band_rows %d, waves %d) is not compiled in",
+                    s.lane_dwords, s.band_rows, s.waves);
+    h->shape = s;
+    h->one_tile = h->x_dw <= 64 * s.lane_dwords && h->y_rows <= s.band_rows * s.waves;
+    int steps = static_cast<int>(c.temporal_steps);
+    if (steps == 0) steps = h->one_tile ? 1024 : 8;
+    if (steps < 1 || steps > 1 << 20) return fail(GOL_ERR_INVALID_ARGUMENT, "temporal_steps out of range");
+    const char* why = nullptr;
+    if (!plan_tiles(s, h->x_dw, h->y_rows, h->one_tile ? 1 : steps, &h->geom, &why))
+        return fail(GOL_ERR_RUNTIME, "temporal_steps %d does not fit the tile: %s", steps, why);
+    h->steps = steps;
+    return GOL_OK;
+}
+
+int enqueue(gol_handle* h, uint64_t generations) {
+    uint64_t left = generations;
+    while (left > 0) {
+        uint32_t* src = h->buf[h->cur];
+        uint32_t* dst = h->buf[h->cur ^ 1];
+        hipError_t e;
+        uint64_t done;
+        if (h->kernel == GOL_KERNEL_PLAIN) {
+            e = launch_plain(static_cast<int>(h->plain_dw), src, dst, h->x_dw, h->y_rows, h->stream);
+            done = 1;
+        } else {
+            done = std::min<uint64_t>(left, static_cast<uint64_t>(h->steps));
+            e = launch_tiles(h->geom, src, dst, h->x_dw, h->y_rows, static_cast<int>(done), h->stream);
+        }
+        if (e != hipSuccess) return fail_hip(e, "kernel launch");
+        h->cur ^= 1;
+        h->launches += 1;
+        h->generations += done;
+        left -= done;
+    }
+    return GOL_OK;
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int now = -1;
+        if (prev >= 0 && hipGetDevice(&now) == hipSuccess && now != prev) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int gol_create(const gol_config* config, gol_handle** out) {
+    clear_error();
+    if (!out) return fail(GOL_ERR_INVALID_ARGUMENT, "out is null");
+    *out = nullptr;
+    int rc = validate_config(config);
+    if (rc) return rc;
+    int dev = config->device;
+    if (dev < 0) {
+        hipError_t e = hipGetDevice(&dev);
+        if (e != hipSuccess) return fail_hip(e, "hipGetDevice");
+    }
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess) return fail_hip(e, "hipGetDeviceCount");
+    if (dev >= count) return fail(GOL_ERR_INVALID_ARGUMENT, "device %d out of range (%d devices)", dev, count);
+
+    gol_handle* h = new gol_handle();
+    h->cfg = *config;
+    h->device = dev;
+    h->x_dw = static_cast<int64_t>(config->x_cells / 32);
+    h->y_rows = static_cast<int64_t>(config->y_cells);
+    h->n_dwords = static_cast<size_t>(h->x_dw) * static_cast<size_t>(h->y_rows);
+    rc = plan(h);
+    if (rc) {
+        delete h;
+        return rc;
+    }
+    DeviceGuard g(dev);
+    const size_t bytes = h->n_dwords * sizeof(uint32_t);
+    if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreate(&h->ev_start)) != hipSuccess || (e = hipEventCreate(&h->ev_stop)) != hipSuccess ||
+        (e = hipMalloc(&h->buf[0], bytes)) != hipSuccess || (e = hipMalloc(&h->buf[1], bytes)) != hipSuccess ||
+        (e = hipMemsetAsync(h->buf[0], 0, bytes, h->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(h->stream)) != hipSuccess) {
+        rc = fail_hip(e, "gol_create");
+        gol_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return GOL_OK;
+}
+
+int gol_upload(gol_handle* h, const void* words, size_t n_words) {
+    clear_error();
+    if (!h || !words) return fail(GOL_ERR_INVALID_ARGUMENT, "null argument");
+    const size_t want = h->n_dwords * 32 / h->cfg.word_bits;
+    if (n_words != want) return fail(GOL_ERR_INVALID_ARGUMENT, "expected %zu words, got %zu", want, n_words);
+    DeviceGuard g(h->device);
+    hipError_t e = hipMemcpyAsync(h->buf[h->cur], words, h->n_dwords * 4, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail_hip(e, "gol_upload");
+    return GOL_OK;
+}
+
+int gol_upload_device(gol_handle* h, const void* device_words, size_t n_words) {
+    clear_error();
+    if (!h || !device_words) return fail(GOL_ERR_INVALID_ARGUMENT, "null argument");
+    const size_t want = h->n_dwords * 32 / h->cfg.word_bits;
+    if (n_words != want) return fail(GOL_ERR_INVALID_ARGUMENT, "expected %zu words, got %zu", want, n_words);
+    DeviceGuard g(h->device);
+    hipError_t e = hipMemcpyAsync(h->buf[h->cur], device_words, h->n_dwords * 4, hipMemcpyDeviceToDevice, h->stream);
+    if (e != hipSuccess) return fail_hip(e, "gol_upload_device");
+    return GOL_OK;
+}
+
+int gol_step(gol_handle* h, uint64_t generations) {
+    clear_error();
+    if (!h) return fail(GOL_ERR_INVALID_ARGUMENT, "null handle");
+    DeviceGuard g(h->device);
+    return enqueue(h, generations);
+}
+
+int gol_step_timed(gol_handle* h, uint64_t generations, double* gpu_ms) {
+    clear_error();
+    if (!h || !gpu_ms) return fail(GOL_ERR_INVALID_ARGUMENT, "null argument");
+    DeviceGuard g(h->device);
+    hipError_t e = hipEventRecord(h->ev_start, h->stream);
+    if (e != hipSuccess) return fail_hip(e, "hipEventRecord");
+    int rc = enqueue(h, generations);
+    if (rc) return rc;
+    if ((e = hipEventRecord(h->ev_stop, h->stream)) != hipSuccess) return fail_hip(e, "hipEventRecord");
+    if ((e = hipEventSynchronize(h->ev_stop)) != hipSuccess) return fail_hip(e, "hipEventSynchronize");
+    float ms = 0.f;
+    if ((e = hipEventElapsedTime(&ms, h->ev_start, h->ev_stop)) != hipSuccess)
+        return fail_hip(e, "hipEventElapsedTime");
+    *gpu_ms = ms;
+    return GOL_OK;
+}
+
+int gol_download(gol_handle* h, void* words, size_t n_words) {
+    clear_error();
+    if (!h || !words) return fail(GOL_ERR_INVALID_ARGUMENT, "null argument");
+    const size_t want = h->n_dwords * 32 / h->cfg.word_bits;
+    if (n_words != want) return fail(GOL_ERR_INVALID_ARGUMENT, "expected %zu words, got %zu", want, n_words);
+    DeviceGuard g(h->device);
+    hipError_t e = hipMemcpyAsync(words, h->buf[h->cur], h->n_dwords * 4, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail_hip(e, "gol_download");
+    return GOL_OK;
+}
+
+int gol_sync(gol_handle* h) {
+    clear_error();
+    if (!h) return fail(GOL_ERR_INVALID_ARGUMENT, "null handle");
+    DeviceGuard g(h->device);
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail_hip(e, "gol_sync");
+    return GOL_OK;
+}
+
+int gol_device_state(gol_handle* h, void** device_words) {
+    clear_error();
+    if (!h || !device_words) return fail(GOL_ERR_INVALID_ARGUMENT, "null argument");
+    *device_words = h->buf[h->cur];
+    return GOL_OK;
+}
+
+int gol_get_stream(gol_handle* h, void** stream) {
+    clear_error();
+    if (!h || !stream) return fail(GOL_ERR_INVALID_ARGUMENT, "null argument");
+    *stream = h->stream;
+    return GOL_OK;
+}
+
+int gol_get_info(gol_handle* h, gol_info* info) {
+    clear_error();
+    if (!h || !info) return fail(GOL_ERR_INVALID_ARGUMENT, "null argument");
+    std::memset(info, 0, sizeof(*info));
+    info->kernel = h->kernel;
+    if (h->kernel == GOL_KERNEL_PLAIN) {
+        info->lane_dwords = h->plain_dw;
+        info->temporal_steps = 1;
+    } else {
+        info->lane_dwords = static_cast<uint32_t>(h->shape.lane_dwords);
+        info->band_rows = static_cast<uint32_t>(h->shape.band_rows);
+        info->waves = static_cast<uint32_t>(h->shape.waves);
+        info->temporal_steps = static_cast<uint32_t>(h->steps);
+        info->tiles = static_cast<uint32_t>(h->geom.tiles_x * h->geom.tiles_y);
+    }
+    info->launches = h->launches;
+    info->generations = h->generations;
+    info->device_bytes = 2 * h->n_dwords * 4;
+    return GOL_OK;
+}
+
+void gol_destroy(gol_handle* h) {
+    if (!h) return;
+    {
+        DeviceGuard g(h->device);
+        if (h->stream) (void)hipStreamSynchronize(h->stream);
+        if (h->buf[0]) (void)hipFree(h->buf[0]);
+        if (h->buf[1]) (void)hipFree(h->buf[1]);
+        if (h->ev_start) (void)hipEventDestroy(h->ev_start);
+        if (h->ev_stop) (void)hipEventDestroy(h->ev_stop);
+        if (h->stream) (void)hipStreamDestroy(h->stream);
+    }
+    delete h;
+}
+
+const char* gol_last_error(void) { return last_error(); }
+
+const char* gol_version(void) { return "golhip 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,20 @@
+// Error plumbing and tile-shape defaults shared by the C ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gol_kernels.hpp"
+
+namespace golhip {
+
+// Record a message for gol_last_error() and return `code`.
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail_hip(hipError_t e, const char* what);
+void clear_error();
+const char* last_error();
+
+TileShape default_shape(int64_t x_dw, int64_t y_rows);
+bool shape_compiled(const TileShape& s);
+
+}  // namespace golhip

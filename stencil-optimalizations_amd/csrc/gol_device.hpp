@@ -1,0 +1,194 @@
+// Device-side bit logic of the B3/S23 stencil on row-packed cells.
+//
+// Layout (shared by rows-64 and rows-32, see below): a grid row of X cells is
+// X/32 little-endian dwords; bit k of dword j is cell 32*j + k. A rows-64 word
+// (wx, y) with bit k = cell 64*wx + k (algorithms/_shared/bitwise/bit_modes.hpp:127-163,
+// general_bit_grid.hpp:217-246) is, in memory, exactly the dword pair
+// (2*wx, 2*wx+1) of this layout, so one kernel serves both word widths.
+//
+// Counting scheme (written for gfx950's 3-input VALU ops: v_xor3_b32,
+// v_bitop3_b32 and the funnel shift v_alignbit_b32):
+//   per row r:  w = west neighbour plane, e = east neighbour plane
+//               H(r) = w + c + e  as two bit planes (h0, h1)     (3-cell sum)
+//               E(r) = w + e      as two bit planes (e0, e1)     (2-cell sum)
+//   per cell:   n = H(r-1) + E(r) + H(r+1)   (eight neighbours, 0..8)
+//               next = (n | alive) == 3
+// Every row's H and E are built once and used by three output rows; the
+// final combine costs six 3-input ops per dword. The reference computes the
+// same rule per word with an explicit 9-word neighbourhood
+// (_shared/bitwise/bitwise-ops/adder.cuh:17-51) or with column sums and lane
+// shuffles (warp-exchange-full-adder-rows.cuh:19-88); this formulation is the
+// row-sum dual of the latter.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace golhip {
+
+__device__ __forceinline__ uint32_t maj3(uint32_t a, uint32_t b, uint32_t c) {
+    return (a & b) | (c & (a | b));
+}
+
+// West neighbour plane of dword c: bit k holds cell k-1; bit 0 takes bit 31 of
+// the dword to the left.
+__device__ __forceinline__ uint32_t west_of(uint32_t left_dword, uint32_t c) {
+    return __builtin_amdgcn_alignbit(c, left_dword, 31);
+}
+
+// East neighbour plane of dword c: bit k holds cell k+1; bit 31 takes bit 0 of
+// the dword to the right.
+__device__ __forceinline__ uint32_t east_of(uint32_t c, uint32_t right_dword) {
+    return __builtin_amdgcn_alignbit(right_dword, c, 1);
+}
+
+struct RowSum {
+    uint32_t h0, h1;  // west + centre + east, bits 0 and 1
+    uint32_t e0, e1;  // west + east, bits 0 and 1
+};
+
+__device__ __forceinline__ RowSum row_sum(uint32_t left_dword, uint32_t c, uint32_t right_dword) {
+    const uint32_t w = west_of(left_dword, c);
+    const uint32_t e = east_of(c, right_dword);
+    RowSum s;
+    s.h0 = w ^ c ^ e;
+    s.h1 = maj3(w, c, e);
+    s.e0 = w ^ e;
+    s.e1 = w & e;
+    return s;
+}
+
+// Only the 3-cell sum (rows above and below the band need no E).
+__device__ __forceinline__ void row_sum_h(uint32_t left_dword, uint32_t c, uint32_t right_dword,
+                                          uint32_t& h0, uint32_t& h1) {
+    const uint32_t w = west_of(left_dword, c);
+    const uint32_t e = east_of(c, right_dword);
+    h0 = w ^ c ^ e;
+    h1 = maj3(w, c, e);
+}
+
+// n = A + E + B with A, B in 0..3 and E in 0..2:
+//   n = l0 + 2*(lc + p) + 4*q,  l0/lc = sum/carry of the low bits, p/q of the high bits.
+// (n | alive) == 3  <=>  (l0 | alive) & (lc + p == 1) & (q == 0).
+__device__ __forceinline__ uint32_t next_state(uint32_t a0, uint32_t a1, uint32_t e0, uint32_t e1,
+                                               uint32_t b0, uint32_t b1, uint32_t alive) {
+    const uint32_t l0 = a0 ^ e0 ^ b0;
+    const uint32_t lc = maj3(a0, e0, b0);
+    const uint32_t p = a1 ^ e1 ^ b1;
+    const uint32_t q = maj3(a1, e1, b1);
+    return (l0 | alive) & (lc ^ p) & ~q;
+}
+
+// Cross-lane word edges over the whole wave64 (DPP wave_shr:1 / wave_shl:1).
+// Lane i receives v from lane i-1; lane 0 receives 0.
+__device__ __forceinline__ uint32_t from_lower_lane(uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x138, 0xf, 0xf, false));
+}
+// Lane i receives v from lane i+1; lane 63 receives 0.
+__device__ __forceinline__ uint32_t from_upper_lane(uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x130, 0xf, 0xf, false));
+}
+// Same, but the lane at the wave edge receives `edge` instead of 0.
+__device__ __forceinline__ uint32_t from_lower_lane_or(uint32_t v, uint32_t edge) {
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(static_cast<int>(edge), static_cast<int>(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ uint32_t from_upper_lane_or(uint32_t v, uint32_t edge) {
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(static_cast<int>(edge), static_cast<int>(v), 0x130, 0xf, 0xf, false));
+}
+
+// Advance a band of BR rows x DW dwords held by one lane (the wave holds BR
+// rows x 64*DW dwords) by one generation, in place.
+//   above/below: the rows just outside the band (same dword columns).
+//   MASK_ROWS: band rows outside [row_lo, row_hi) (wave-uniform) lie outside
+//   the grid and are cleared after the update (dead boundary).
+//   MASK_COLS: dwords with colmask 0 lie outside the grid and are cleared.
+//   EDGE_IN: wave_edge_l/r[i] hold the dword just left of lane 0's first dword
+//   / right of lane 63's last dword for rows above (i=0), band (1..BR) and
+//   below (BR+1); only lanes 0 / 63 read them. Without EDGE_IN the wave edges
+//   see dead cells (a tile edge, covered by the tile's halo).
+//
+// Columns are processed left to right, each from the top row down, keeping
+// the running row sums of one column. old[] holds the previous column's
+// pre-update values (west neighbours). Lane edges need the neighbouring lane's
+// pre-update values: the west edge (left lane's last column) is read while the
+// first column is processed, before that lane writes its last column; the east
+// edge (right lane's first column) is written during the first column pass, so
+// bit 0 of each of its rows is packed into one dword before any write.
+template <int DW, int BR, bool MASK_ROWS, bool MASK_COLS, bool EDGE_IN = false>
+__device__ __forceinline__ void advance_band(uint32_t (&s)[BR][DW], const uint32_t (&above)[DW],
+                                             const uint32_t (&below)[DW], int row_lo, int row_hi,
+                                             const uint32_t (&colmask)[DW],
+                                             const uint32_t* wave_edge_l = nullptr,
+                                             const uint32_t* wave_edge_r = nullptr) {
+    static_assert(DW == 1 || BR + 2 <= 32, "east-edge bits of a band are packed into one dword");
+    auto edge_from_left = [&](uint32_t v, int i) -> uint32_t {
+        return EDGE_IN ? from_lower_lane_or(v, wave_edge_l[i]) : from_lower_lane(v);
+    };
+    auto edge_from_right = [&](uint32_t v, int i) -> uint32_t {
+        return EDGE_IN ? from_upper_lane_or(v, wave_edge_r[i]) : from_upper_lane(v);
+    };
+
+    // East edges. For DW == 1 the first column is also the last, so they are
+    // read lazily like the west edges; otherwise bit 0 of row i (0 = above,
+    // BR+1 = below) of the right lane's first column lands in bit i.
+    uint32_t east_bits = 0;
+    if (DW > 1) {
+        uint32_t pack = above[0] & 1u;
+#pragma unroll
+        for (int r = 0; r < BR; ++r) pack |= (s[r][0] & 1u) << (r + 1);
+        pack |= (below[0] & 1u) << (BR + 1);
+        uint32_t edge_pack = 0;
+        if (EDGE_IN) {
+            // pack the wave-edge dwords the same way for lane 63
+#pragma unroll
+            for (int i = 0; i < BR + 2; ++i) edge_pack |= (wave_edge_r[i] & 1u) << i;
+            east_bits = from_upper_lane_or(pack, edge_pack);
+        } else {
+            east_bits = from_upper_lane(pack);
+        }
+    }
+
+    uint32_t old[BR];
+#pragma unroll
+    for (int d = 0; d < DW; ++d) {
+        const uint32_t al = d == 0 ? edge_from_left(above[DW - 1], 0) : above[d - 1];
+        const uint32_t ar = d < DW - 1 ? above[d + 1] : (DW == 1 ? edge_from_right(above[0], 0) : east_bits);
+        uint32_t pa0, pa1;  // H of the row above the current one
+        row_sum_h(al, above[d], ar, pa0, pa1);
+
+        const uint32_t cl = d == 0 ? edge_from_left(s[0][DW - 1], 1) : old[0];
+        const uint32_t cr = d < DW - 1 ? s[0][d + 1] : (DW == 1 ? edge_from_right(s[0][0], 1) : east_bits >> 1);
+        RowSum cur = row_sum(cl, s[0][d], cr);
+#pragma unroll
+        for (int r = 0; r < BR; ++r) {
+            RowSum nxt;
+            if (r + 1 < BR) {
+                const uint32_t nl = d == 0 ? edge_from_left(s[r + 1][DW - 1], r + 2) : old[r + 1];
+                const uint32_t nr = d < DW - 1 ? s[r + 1][d + 1]
+                                               : (DW == 1 ? edge_from_right(s[r + 1][0], r + 2) : east_bits >> (r + 2));
+                nxt = row_sum(nl, s[r + 1][d], nr);
+            } else {
+                const uint32_t nl = d == 0 ? edge_from_left(below[DW - 1], BR + 1) : below[d - 1];
+                const uint32_t nr = d < DW - 1 ? below[d + 1]
+                                               : (DW == 1 ? edge_from_right(below[0], BR + 1) : east_bits >> (BR + 1));
+                row_sum_h(nl, below[d], nr, nxt.h0, nxt.h1);
+                nxt.e0 = nxt.e1 = 0;
+            }
+            uint32_t v = next_state(pa0, pa1, cur.e0, cur.e1, nxt.h0, nxt.h1, s[r][d]);
+            if (MASK_COLS) v &= colmask[d];
+            if (MASK_ROWS) v = (r >= row_lo && r < row_hi) ? v : 0u;
+            pa0 = cur.h0;
+            pa1 = cur.h1;
+            cur = nxt;
+            old[r] = s[r][d];
+            s[r][d] = v;
+            // keep the scheduler from hoisting later rows' cross-lane reads
+            // (it would hold every edge of the band live at once)
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
+}  // namespace golhip

@@ -1,0 +1,326 @@
+// HIP kernels for gfx950: the plain one-generation stencil and the
+// temporal-tile stencil (T generations per launch).
+//
+// Reference counterparts:
+//   plain kernel     algorithms/cuda-naive-bitwise/cuda_naive_bitwise_kernel.cu:32-79 and
+//                    algorithms/cuda-simple-warp-exchange/simple_warp_exchange.cu:33-85
+//                    (one launch per generation, input/output ping-pong)
+//   tile kernel      algorithms/cuda-temporal-rowed/temporal_rowed.cu:33-229
+//                    (T generations per launch on a tile with a T-row halo)
+//
+// MI355X mapping. A wavefront holds a band of BR rows x 64*DW dwords in VGPRs
+// (lane l owns dwords [l*DW, (l+1)*DW) of each row). Word edges between lanes
+// travel by DPP wave shifts; the rows just above and below a band come from the
+// neighbouring wavefronts through a double-buffered LDS slot, so a generation
+// costs one workgroup barrier. A tile is NW bands stacked vertically. Tiles
+// overlap by their halo: halo_rows rows (>= generations per launch) above and
+// below, and one lane (32*DW cells >= generations per launch) left and right,
+// and neither halo is needed along an axis where one tile spans the grid.
+// Cells outside the grid are cleared after every generation, which is the
+// reference's dead boundary (algorithms/cpu-naive/gol_cpu_naive.hpp:92-117).
+
+#include "gol_device.hpp"
+#include "gol_kernels.hpp"
+
+namespace golhip {
+
+namespace {
+
+// Loads are branch-free: the address is clamped into the grid and the value
+// selected afterwards, so the register band never flows through a branch.
+// With vec_ok (x_dw and gx0 multiples of DW) a lane's segment lies wholly
+// inside or wholly outside the row.
+template <int DW>
+__device__ __forceinline__ void load_seg(const uint32_t* __restrict__ src, int x_dw, int y_rows, int gy, int gx0,
+                                         bool vec_ok, uint32_t (&out)[DW]) {
+    const bool row_in = gy >= 0 && gy < y_rows;
+    const int cy = gy < 0 ? 0 : (gy >= y_rows ? y_rows - 1 : gy);
+    const uint32_t* row = src + static_cast<int64_t>(cy) * x_dw;
+    if (vec_ok) {
+        const bool in = row_in && gx0 >= 0 && gx0 + DW <= x_dw;
+        const int cx = gx0 < 0 ? 0 : (gx0 + DW > x_dw ? x_dw - DW : gx0);
+        if constexpr (DW == 1) {
+            const uint32_t v = row[cx];
+            out[0] = in ? v : 0u;
+        } else if constexpr (DW == 2) {
+            const uint2 v = *reinterpret_cast<const uint2*>(row + cx);
+            out[0] = in ? v.x : 0u;
+            out[1] = in ? v.y : 0u;
+        } else {
+#pragma unroll
+            for (int k = 0; k < DW / 4; ++k) {
+                const uint4 v = reinterpret_cast<const uint4*>(row + cx)[k];
+                out[4 * k + 0] = in ? v.x : 0u;
+                out[4 * k + 1] = in ? v.y : 0u;
+                out[4 * k + 2] = in ? v.z : 0u;
+                out[4 * k + 3] = in ? v.w : 0u;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < DW; ++d) {
+            const int gx = gx0 + d;
+            const bool in = row_in && gx >= 0 && gx < x_dw;
+            const uint32_t v = row[gx < 0 ? 0 : (gx >= x_dw ? x_dw - 1 : gx)];
+            out[d] = in ? v : 0u;
+        }
+    }
+}
+
+template <int DW>
+__device__ __forceinline__ void store_seg(uint32_t* __restrict__ dst, int x_dw, int gy, int gx0, bool vec_ok,
+                                          const uint32_t (&v)[DW]) {
+    uint32_t* row = dst + static_cast<int64_t>(gy) * x_dw;
+    if (vec_ok && gx0 >= 0 && gx0 + DW <= x_dw) {
+        if constexpr (DW == 1) {
+            row[gx0] = v[0];
+        } else if constexpr (DW == 2) {
+            *reinterpret_cast<uint2*>(row + gx0) = make_uint2(v[0], v[1]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < DW / 4; ++k) {
+                reinterpret_cast<uint4*>(row + gx0)[k] =
+                    make_uint4(v[4 * k + 0], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < DW; ++d) {
+            const int gx = gx0 + d;
+            if (gx >= 0 && gx < x_dw) row[gx] = v[d];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain kernel: each lane owns PLAIN_RB rows x DW dwords and reads the rows
+// just above and below its strip from global memory; lanes 0 and 63 also load
+// the dword beyond the wavefront's edge (the DPP shift has no source there).
+// ---------------------------------------------------------------------------
+constexpr int PLAIN_RB = 8;
+constexpr int PLAIN_WAVES = 4;
+
+template <int DW>
+__global__ __launch_bounds__(64 * PLAIN_WAVES) void plain_kernel(const uint32_t* __restrict__ src,
+                                                                 uint32_t* __restrict__ dst, int x_dw, int y_rows) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+    const int wave_x0 = blockIdx.x * (64 * DW);
+    const int gx0 = wave_x0 + lane * DW;
+    const int gy0 = (blockIdx.y * PLAIN_WAVES + w) * PLAIN_RB;
+    if (gy0 >= y_rows) return;  // whole wavefront leaves together
+    const bool vec_ok = (x_dw % DW) == 0;
+
+    uint32_t s[PLAIN_RB][DW], above[DW], below[DW];
+    load_seg<DW>(src, x_dw, y_rows, gy0 - 1, gx0, vec_ok, above);
+#pragma unroll
+    for (int r = 0; r < PLAIN_RB; ++r) load_seg<DW>(src, x_dw, y_rows, gy0 + r, gx0, vec_ok, s[r]);
+    load_seg<DW>(src, x_dw, y_rows, gy0 + PLAIN_RB, gx0, vec_ok, below);
+
+    // One load per row fetches both wave-edge dwords: lane 0 the dword left of
+    // the wavefront, lane 63 the dword right of it (other lanes re-read their own).
+    uint32_t el[PLAIN_RB + 2], er[PLAIN_RB + 2];
+    const int xe = lane == 0 ? wave_x0 - 1 : (lane == 63 ? wave_x0 + 64 * DW : gx0);
+    const bool xe_in = xe >= 0 && xe < x_dw;
+    const int cxe = xe < 0 ? 0 : (xe >= x_dw ? x_dw - 1 : xe);
+#pragma unroll
+    for (int i = 0; i < PLAIN_RB + 2; ++i) {
+        const int gy = gy0 - 1 + i;
+        const bool in = xe_in && gy >= 0 && gy < y_rows;
+        const int cy = gy < 0 ? 0 : (gy >= y_rows ? y_rows - 1 : gy);
+        const uint32_t v = src[static_cast<int64_t>(cy) * x_dw + cxe];
+        el[i] = (in && lane == 0) ? v : 0u;
+        er[i] = (in && lane == 63) ? v : 0u;
+    }
+    uint32_t colmask[DW];
+#pragma unroll
+    for (int d = 0; d < DW; ++d) colmask[d] = ~0u;
+    advance_band<DW, PLAIN_RB, false, false, true>(s, above, below, 0, PLAIN_RB, colmask, el, er);
+
+#pragma unroll
+    for (int r = 0; r < PLAIN_RB; ++r) {
+        if (gy0 + r < y_rows) store_seg<DW>(dst, x_dw, gy0 + r, gx0, vec_ok, s[r]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Temporal tile kernel.
+// ---------------------------------------------------------------------------
+struct TileArgs {
+    int x_dw, y_rows, gens;
+    int halo_rows, halo_lanes;
+    int stride_rows, stride_dwords, tiles_x;
+};
+
+template <int DW, int BR, int NW, bool MASK_ROWS, bool MASK_COLS>
+__device__ __forceinline__ void run_generations(uint32_t (&s)[BR][DW], uint32_t (*slots)[NW][2][64 * DW], int gens,
+                                                int lane, int w, int row_lo, int row_hi,
+                                                const uint32_t (&colmask)[DW]) {
+    for (int g = 0; g < gens; ++g) {
+        const int par = g & 1;
+#pragma unroll
+        for (int d = 0; d < DW; ++d) {
+            slots[par][w][0][lane * DW + d] = s[0][d];
+            slots[par][w][1][lane * DW + d] = s[BR - 1][d];
+        }
+        __syncthreads();
+        uint32_t above[DW], below[DW];
+#pragma unroll
+        for (int d = 0; d < DW; ++d) {
+            above[d] = w > 0 ? slots[par][w - 1][1][lane * DW + d] : 0u;
+            below[d] = w < NW - 1 ? slots[par][w + 1][0][lane * DW + d] : 0u;
+        }
+        advance_band<DW, BR, MASK_ROWS, MASK_COLS>(s, above, below, row_lo, row_hi, colmask);
+    }
+}
+
+template <int DW, int BR, int NW>
+__global__ __launch_bounds__(64 * NW) void tile_kernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                                       TileArgs a) {
+    __shared__ uint32_t slots[2][NW][2][64 * DW];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+    const int tx = blockIdx.x % a.tiles_x;
+    const int ty = blockIdx.x / a.tiles_x;
+    const int x0 = tx * a.stride_dwords - a.halo_lanes * DW;
+    const int y0 = ty * a.stride_rows - a.halo_rows;
+    const int gx0 = x0 + lane * DW;
+    const int gy0 = y0 + w * BR;
+    const bool vec_ok = (a.x_dw % DW) == 0;
+
+    uint32_t s[BR][DW];
+#pragma unroll
+    for (int r = 0; r < BR; ++r) load_seg<DW>(src, a.x_dw, a.y_rows, gy0 + r, gx0, vec_ok, s[r]);
+
+    uint32_t colmask[DW];
+#pragma unroll
+    for (int d = 0; d < DW; ++d) colmask[d] = (gx0 + d >= 0 && gx0 + d < a.x_dw) ? ~0u : 0u;
+    // band rows [row_lo, row_hi) lie inside the grid (wave-uniform)
+    const int row_lo = gy0 < 0 ? -gy0 : 0;
+    const int row_hi = a.y_rows - gy0 < BR ? a.y_rows - gy0 : BR;
+
+    const bool edge_y = y0 < 0 || y0 + NW * BR > a.y_rows;  // workgroup-uniform
+    const bool edge_x = x0 < 0 || x0 + 64 * DW > a.x_dw;
+    if (edge_x) {
+        run_generations<DW, BR, NW, true, true>(s, slots, a.gens, lane, w, row_lo, row_hi, colmask);
+    } else if (edge_y) {
+        run_generations<DW, BR, NW, true, false>(s, slots, a.gens, lane, w, row_lo, row_hi, colmask);
+    } else {
+        run_generations<DW, BR, NW, false, false>(s, slots, a.gens, lane, w, row_lo, row_hi, colmask);
+    }
+
+    const int tile_rows = NW * BR;
+    const bool lane_inside = lane >= a.halo_lanes && lane < 64 - a.halo_lanes;
+#pragma unroll
+    for (int r = 0; r < BR; ++r) {
+        const int tr = w * BR + r;
+        const int gy = gy0 + r;
+        if (tr >= a.halo_rows && tr < tile_rows - a.halo_rows && gy >= 0 && gy < a.y_rows && lane_inside) {
+            store_seg<DW>(dst, a.x_dw, gy, gx0, vec_ok, s[r]);
+        }
+    }
+}
+
+template <int DW, int BR, int NW>
+hipError_t launch_tile_shape(const TileGeometry& g, const uint32_t* src, uint32_t* dst, int x_dw, int y_rows, int gens,
+                             hipStream_t stream) {
+    TileArgs a;
+    a.x_dw = x_dw;
+    a.y_rows = y_rows;
+    a.gens = gens;
+    a.halo_rows = g.halo_rows;
+    a.halo_lanes = g.halo_lanes;
+    a.stride_rows = g.stride_rows;
+    a.stride_dwords = g.stride_dwords;
+    a.tiles_x = g.tiles_x;
+    const unsigned blocks = static_cast<unsigned>(g.tiles_x) * static_cast<unsigned>(g.tiles_y);
+    hipLaunchKernelGGL((tile_kernel<DW, BR, NW>), dim3(blocks), dim3(64 * NW), 0, stream, src, dst, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+const TileShape kTileShapes[] = {
+    {8, 8, 8}, {8, 4, 16}, {4, 8, 8}, {4, 8, 16}, {2, 16, 8}, {1, 16, 8}, {1, 32, 16},
+};
+const int kNumTileShapes = static_cast<int>(sizeof(kTileShapes) / sizeof(kTileShapes[0]));
+
+bool plan_tiles(const TileShape& shape, int64_t x_dw, int64_t y_rows, int max_gens, TileGeometry* g,
+                const char** why) {
+    const int DW = shape.lane_dwords, BR = shape.band_rows, NW = shape.waves;
+    g->lane_dwords = DW;
+    g->band_rows = BR;
+    g->waves = NW;
+    g->tile_rows = NW * BR;
+    g->tile_dwords = 64 * DW;
+    if (x_dw <= g->tile_dwords) {
+        g->halo_lanes = 0;
+        g->stride_dwords = g->tile_dwords;
+        g->tiles_x = 1;
+    } else {
+        if (max_gens > 32 * DW) {
+            *why = "generations per launch exceed the one-lane x halo (32 * lane_dwords cells)";
+            return false;
+        }
+        g->halo_lanes = 1;
+        g->stride_dwords = (64 - 2) * DW;
+        g->tiles_x = static_cast<int>((x_dw + g->stride_dwords - 1) / g->stride_dwords);
+    }
+    if (y_rows <= g->tile_rows) {
+        g->halo_rows = 0;
+        g->stride_rows = g->tile_rows;
+        g->tiles_y = 1;
+    } else {
+        if (g->tile_rows - 2 * max_gens < 1) {
+            *why = "generations per launch leave no interior rows in the tile";
+            return false;
+        }
+        g->halo_rows = max_gens;
+        g->stride_rows = g->tile_rows - 2 * max_gens;
+        g->tiles_y = static_cast<int>((y_rows + g->stride_rows - 1) / g->stride_rows);
+    }
+    return true;
+}
+
+hipError_t launch_tiles(const TileGeometry& g, const uint32_t* src, uint32_t* dst, int64_t x_dw, int64_t y_rows,
+                        int gens, hipStream_t stream) {
+    const int xd = static_cast<int>(x_dw), yr = static_cast<int>(y_rows);
+#define GOLHIP_SHAPE(DW, BR, NW)                                                      \
+    if (g.lane_dwords == DW && g.band_rows == BR && g.waves == NW)                    \
+        return launch_tile_shape<DW, BR, NW>(g, src, dst, xd, yr, gens, stream);
+    GOLHIP_SHAPE(8, 8, 8)
+    GOLHIP_SHAPE(8, 4, 16)
+    GOLHIP_SHAPE(4, 8, 8)
+    GOLHIP_SHAPE(4, 8, 16)
+    GOLHIP_SHAPE(2, 16, 8)
+    GOLHIP_SHAPE(1, 16, 8)
+    GOLHIP_SHAPE(1, 32, 16)
+#undef GOLHIP_SHAPE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_plain(int lane_dwords, const uint32_t* src, uint32_t* dst, int64_t x_dw, int64_t y_rows,
+                        hipStream_t stream) {
+    const int xd = static_cast<int>(x_dw), yr = static_cast<int>(y_rows);
+    const unsigned gy = static_cast<unsigned>((y_rows + PLAIN_WAVES * PLAIN_RB - 1) / (PLAIN_WAVES * PLAIN_RB));
+    auto gx = [&](int dw) { return static_cast<unsigned>((x_dw + 64 * dw - 1) / (64 * dw)); };
+    switch (lane_dwords) {
+        case 1:
+            hipLaunchKernelGGL(plain_kernel<1>, dim3(gx(1), gy), dim3(64 * PLAIN_WAVES), 0, stream, src, dst, xd, yr);
+            break;
+        case 2:
+            hipLaunchKernelGGL(plain_kernel<2>, dim3(gx(2), gy), dim3(64 * PLAIN_WAVES), 0, stream, src, dst, xd, yr);
+            break;
+        case 4:
+            hipLaunchKernelGGL(plain_kernel<4>, dim3(gx(4), gy), dim3(64 * PLAIN_WAVES), 0, stream, src, dst, xd, yr);
+            break;
+        case 8:
+            hipLaunchKernelGGL(plain_kernel<8>, dim3(gx(8), gy), dim3(64 * PLAIN_WAVES), 0, stream, src, dst, xd, yr);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace golhip

@@ -1,0 +1,45 @@
+// Launch-side view of the stencil kernels (host code only sees this).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace golhip {
+
+// Geometry of one temporal-tile launch over one grid.
+struct TileGeometry {
+    int lane_dwords;  // DW: dwords per lane (32*DW cells per lane, 64 lanes per wave)
+    int band_rows;    // BR: rows per wavefront
+    int waves;        // NW: wavefronts per workgroup, stacked vertically
+    int tile_rows;    // NW*BR
+    int tile_dwords;  // 64*DW
+    int halo_rows;    // rows of halo above and below (0 when one tile spans the grid height)
+    int halo_lanes;   // lanes of halo left and right (0 when one tile spans the grid width)
+    int stride_rows;  // interior rows per tile
+    int stride_dwords;// interior dwords per tile
+    int tiles_x, tiles_y;
+};
+
+// Tile kernel shapes compiled into the library: (DW, BR, NW).
+struct TileShape {
+    int lane_dwords, band_rows, waves;
+};
+extern const TileShape kTileShapes[];
+extern const int kNumTileShapes;
+
+// Compute the geometry for a grid of x_dw dwords x y_rows rows advanced by up
+// to max_gens generations per launch. Returns false (with a reason) when the
+// shape cannot cover max_gens generations.
+bool plan_tiles(const TileShape& shape, int64_t x_dw, int64_t y_rows, int max_gens, TileGeometry* g,
+                const char** why);
+
+// One launch: advance src by gens (<= the planned max_gens) generations into dst.
+hipError_t launch_tiles(const TileGeometry& g, const uint32_t* src, uint32_t* dst, int64_t x_dw, int64_t y_rows,
+                        int gens, hipStream_t stream);
+
+// Plain kernel: one generation per launch, global memory to global memory.
+// lane_dwords in {1,2,4,8}.
+hipError_t launch_plain(int lane_dwords, const uint32_t* src, uint32_t* dst, int64_t x_dw, int64_t y_rows,
+                        hipStream_t stream);
+
+}  // namespace golhip

@@ -1,0 +1,234 @@
+"""ctypes binding of the golhip C ABI (include/golhip.h).
+
+This is the Python-side stub a reference maintainer would write to drive the
+engine from a script; it mirrors the C calls one to one and raises
+``GolError`` carrying the C status code. It loads ``libgolhip.so`` from this
+directory and fails loudly when the library has not been built: there is no
+CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgolhip.so")
+
+GOL_OK = 0
+GOL_ERR_INVALID_ARGUMENT = 1
+GOL_ERR_RUNTIME = 2
+GOL_ERR_HIP = 3
+GOL_ERR_OUT_OF_MEMORY = 4
+GOL_ERR_NOT_FOUND = 5
+GOL_ERR_BUSY = 6
+
+KERNEL_AUTO = 0
+KERNEL_PLAIN = 1
+KERNEL_TEMPORAL = 2
+KERNEL_ACTIVE = 3
+
+# every symbol include/golhip.h declares
+EXPORTED_SYMBOLS = (
+    "gol_create", "gol_upload", "gol_upload_device", "gol_step", "gol_step_timed", "gol_download", "gol_sync",
+    "gol_device_state", "gol_get_stream", "gol_get_info", "gol_destroy", "gol_last_error",
+    "gol_version", "gol_fill_random", "gol_fill_bernoulli", "gol_fill_patches",
+)
+
+
+class GolError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"golhip error {code}: {message}")
+        self.code = code
+
+
+class GolConfig(ctypes.Structure):
+    _fields_ = [
+        ("x_cells", ctypes.c_uint64),
+        ("y_cells", ctypes.c_uint64),
+        ("word_bits", ctypes.c_uint32),
+        ("kernel", ctypes.c_uint32),
+        ("temporal_steps", ctypes.c_uint32),
+        ("lane_dwords", ctypes.c_uint32),
+        ("band_rows", ctypes.c_uint32),
+        ("waves", ctypes.c_uint32),
+        ("device", ctypes.c_int32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class GolInfo(ctypes.Structure):
+    _fields_ = [
+        ("kernel", ctypes.c_uint32),
+        ("lane_dwords", ctypes.c_uint32),
+        ("band_rows", ctypes.c_uint32),
+        ("waves", ctypes.c_uint32),
+        ("temporal_steps", ctypes.c_uint32),
+        ("tiles", ctypes.c_uint32),
+        ("launches", ctypes.c_uint64),
+        ("generations", ctypes.c_uint64),
+        ("last_active_tiles", ctypes.c_uint64),
+        ("device_bytes", ctypes.c_uint64),
+    ]
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libgolhip.so (raises if it is missing: build it first)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is missing: build it with __graft_entry__.build() (no CPU fallback exists)")
+    lib = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    lib.gol_create.argtypes = [ctypes.POINTER(GolConfig), ctypes.POINTER(vp)]
+    lib.gol_upload.argtypes = [vp, vp, ctypes.c_size_t]
+    lib.gol_upload_device.argtypes = [vp, vp, ctypes.c_size_t]
+    lib.gol_step.argtypes = [vp, ctypes.c_uint64]
+    lib.gol_step_timed.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
+    lib.gol_download.argtypes = [vp, vp, ctypes.c_size_t]
+    lib.gol_sync.argtypes = [vp]
+    lib.gol_device_state.argtypes = [vp, ctypes.POINTER(vp)]
+    lib.gol_get_stream.argtypes = [vp, ctypes.POINTER(vp)]
+    lib.gol_get_info.argtypes = [vp, ctypes.POINTER(GolInfo)]
+    lib.gol_destroy.argtypes = [vp]
+    lib.gol_destroy.restype = None
+    lib.gol_last_error.restype = ctypes.c_char_p
+    lib.gol_version.restype = ctypes.c_char_p
+    u64 = ctypes.c_uint64
+    lib.gol_fill_random.argtypes = [vp, u64, u64, ctypes.c_uint32, u64]
+    lib.gol_fill_bernoulli.argtypes = [vp, u64, u64, ctypes.c_uint32, u64, ctypes.c_double]
+    lib.gol_fill_patches.argtypes = [vp, u64, u64, ctypes.c_uint32, u64, ctypes.c_uint32,
+                                     ctypes.c_uint32, ctypes.c_double]
+    for name in ("gol_create", "gol_upload", "gol_upload_device", "gol_step", "gol_step_timed", "gol_download",
+                 "gol_sync", "gol_device_state", "gol_get_stream", "gol_get_info",
+                 "gol_fill_random", "gol_fill_bernoulli", "gol_fill_patches"):
+        getattr(lib, name).restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != GOL_OK:
+        msg = load_library().gol_last_error().decode(errors="replace")
+        raise GolError(rc, msg)
+
+
+class Grid:
+    """One grid's device state: gol_create / upload / step / download / destroy."""
+
+    def __init__(self, x_cells: int, y_cells: int, word_bits: int = 64, kernel: int = KERNEL_AUTO,
+                 temporal_steps: int = 0, lane_dwords: int = 0, band_rows: int = 0, waves: int = 0,
+                 device: int = -1):
+        self.lib = load_library()
+        cfg = GolConfig(x_cells, y_cells, word_bits, kernel, temporal_steps, lane_dwords, band_rows,
+                        waves, device, 0)
+        h = ctypes.c_void_p()
+        _check(self.lib.gol_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+        self.x_cells, self.y_cells, self.word_bits = x_cells, y_cells, word_bits
+        self.dtype = np.uint64 if word_bits == 64 else np.uint32
+        self.n_words = (x_cells // word_bits) * y_cells
+
+    def upload(self, words: np.ndarray) -> None:
+        words = np.ascontiguousarray(words, dtype=self.dtype).reshape(-1)
+        _check(self.lib.gol_upload(self.handle, words.ctypes.data, words.size))
+
+    def upload_device(self, device_words: int) -> None:
+        _check(self.lib.gol_upload_device(self.handle, ctypes.c_void_p(device_words), self.n_words))
+
+    def device_state(self) -> int:
+        p = ctypes.c_void_p()
+        _check(self.lib.gol_device_state(self.handle, ctypes.byref(p)))
+        return p.value or 0
+
+    def step(self, generations: int) -> None:
+        _check(self.lib.gol_step(self.handle, generations))
+
+    def step_timed(self, generations: int) -> float:
+        ms = ctypes.c_double()
+        _check(self.lib.gol_step_timed(self.handle, generations, ctypes.byref(ms)))
+        return ms.value
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.n_words, dtype=self.dtype)
+        _check(self.lib.gol_download(self.handle, out.ctypes.data, out.size))
+        return out
+
+    def sync(self) -> None:
+        _check(self.lib.gol_sync(self.handle))
+
+    def info(self) -> GolInfo:
+        info = GolInfo()
+        _check(self.lib.gol_get_info(self.handle, ctypes.byref(info)))
+        return info
+
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        _check(self.lib.gol_get_stream(self.handle, ctypes.byref(s)))
+        return s.value or 0
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.gol_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _words(x_cells: int, y_cells: int, word_bits: int) -> np.ndarray:
+    return np.zeros((x_cells // word_bits) * y_cells, dtype=np.uint64 if word_bits == 64 else np.uint32)
+
+
+def fill_random(x_cells: int, y_cells: int, seed: int = 42, word_bits: int = 64) -> np.ndarray:
+    """RandomOnesZerosDataLoader (data_loader.hpp:57-75), packed into row words."""
+    w = _words(x_cells, y_cells, word_bits)
+    _check(load_library().gol_fill_random(w.ctypes.data, x_cells, y_cells, word_bits, seed))
+    return w
+
+
+def fill_bernoulli(x_cells: int, y_cells: int, seed: int, p: float, word_bits: int = 64) -> np.ndarray:
+    w = _words(x_cells, y_cells, word_bits)
+    _check(load_library().gol_fill_bernoulli(w.ctypes.data, x_cells, y_cells, word_bits, seed, p))
+    return w
+
+
+def fill_patches(x_cells: int, y_cells: int, seed: int, n_patches: int, patch: int, p: float,
+                 word_bits: int = 64) -> np.ndarray:
+    w = _words(x_cells, y_cells, word_bits)
+    _check(load_library().gol_fill_patches(w.ctypes.data, x_cells, y_cells, word_bits, seed,
+                                           n_patches, patch, p))
+    return w
+
+
+def pack_rows(cells: np.ndarray, word_bits: int = 64) -> np.ndarray:
+    """Pack a (Y, X) 0/1 array into row words, bit k of word wx = cell word_bits*wx + k
+    (src/algorithms/_shared/bitwise/general_bit_grid.hpp:217-246, FullyPackedRows)."""
+    y, x = cells.shape
+    if x % word_bits:
+        raise ValueError(f"Grid dimensions X must be a multiple of {word_bits}")
+    bits = np.packbits((cells != 0).astype(np.uint8), axis=1, bitorder="little")
+    return bits.view(np.uint64 if word_bits == 64 else np.uint32).reshape(y, x // word_bits).copy()
+
+
+def unpack_rows(words: np.ndarray, x_cells: int, y_cells: int) -> np.ndarray:
+    """Inverse of pack_rows: (Y, X) uint8 array of 0/1 cells."""
+    w = np.ascontiguousarray(words).reshape(y_cells, -1)
+    return np.unpackbits(w.view(np.uint8), axis=1, bitorder="little")[:, :x_cells]

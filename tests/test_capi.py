@@ -1,0 +1,92 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every symbol the
+header declares, validates configurations before touching a device, and its
+host-side input generators equal the oracle's loaders."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import golhip
+import oracle_lib as ol
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "golhip.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gol_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = golhip.load_library()
+    names = declared_symbols()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(golhip.EXPORTED_SYMBOLS) == names
+
+
+def test_version_string():
+    assert b"gfx950" in golhip.load_library().gol_version()
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(x_cells=100, y_cells=64, word_bits=64), golhip.GOL_ERR_INVALID_ARGUMENT),
+    (dict(x_cells=64, y_cells=64, word_bits=16), golhip.GOL_ERR_INVALID_ARGUMENT),
+    (dict(x_cells=0, y_cells=64, word_bits=64), golhip.GOL_ERR_INVALID_ARGUMENT),
+    (dict(x_cells=64, y_cells=64, word_bits=64, kernel=9), golhip.GOL_ERR_INVALID_ARGUMENT),
+    (dict(x_cells=64, y_cells=64, word_bits=64, lane_dwords=3), golhip.GOL_ERR_INVALID_ARGUMENT),
+])
+def test_config_errors_before_device(kw, code):
+    # invalid configurations fail in validation, before any HIP call
+    with pytest.raises(golhip.GolError) as e:
+        golhip.Grid(**kw)
+    assert e.value.code == code
+
+
+def test_error_message_names_the_multiple():
+    with pytest.raises(golhip.GolError) as e:
+        golhip.Grid(96, 8, word_bits=64)
+    assert "multiple of 64" in str(e.value)
+
+
+@pytest.mark.parametrize("x,y,seed", [(64, 3, 42), (2048, 16, 42), (96, 5, 7)])
+def test_fill_random_matches_loader(x, y, seed):
+    wb = 64 if x % 64 == 0 else 32
+    got = golhip.fill_random(x, y, seed, wb)
+    assert np.array_equal(got, ol.pack(ol.random_cells(x, y, seed), wb))
+
+
+def test_fill_random_matches_loader_fixture():
+    f = ol.random_cells(2048, 2048, 42)
+    assert ol.sha256(ol.unpack(golhip.fill_random(2048, 2048, 42), 2048, 2048)) == ol.sha256(f)
+
+
+@pytest.mark.parametrize("p", [0.1, 0.35, 0.5])
+def test_fill_bernoulli_matches_oracle(p):
+    got = golhip.fill_bernoulli(128, 40, 99, p)
+    assert np.array_equal(got, ol.pack(ol.bernoulli_cells(128, 40, 99, p)))
+
+
+def test_fill_patches_places_patches():
+    w = golhip.fill_patches(512, 256, 5, 3, 64, 0.35)
+    cells = ol.unpack(w, 512, 256)
+    assert 0 < cells.sum() <= 3 * 64 * 64
+    # deterministic for a seed, different across seeds
+    assert np.array_equal(golhip.fill_patches(512, 256, 5, 3, 64, 0.35), w)
+    assert not np.array_equal(golhip.fill_patches(512, 256, 6, 3, 64, 0.35), w)
+    # density inside a lone patch is close to p
+    one = ol.unpack(golhip.fill_patches(512, 256, 8, 1, 64, 0.35), 512, 256)
+    assert 0.25 < one.sum() / (64 * 64) < 0.45
+
+
+def test_python_pack_matches_oracle():
+    c = ol.bernoulli_cells(192, 7, 3, 0.5)
+    assert np.array_equal(golhip.pack_rows(c, 64).reshape(-1), ol.pack(c, 64))
+    assert np.array_equal(golhip.pack_rows(c, 32).reshape(-1), ol.pack(c, 32))
+    assert np.array_equal(golhip.unpack_rows(ol.pack(c, 64), 192, 7), c)

@@ -1,0 +1,168 @@
+"""GPU parity: the HIP engine, called through the C ABI, against the oracle.
+
+Bit-exact everywhere (integer bit logic, no tolerance). Small cases compare
+with the naive oracle on the same seeded input; full benchmark sizes compare
+with fixtures written by the oracle's bit-parallel engine
+(tests/golden/make_fixtures.py) and with size-independent properties.
+"""
+
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import golhip
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+FIX = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "fixtures.json")))
+
+KERNELS = [
+    dict(kernel=golhip.KERNEL_PLAIN),
+    dict(kernel=golhip.KERNEL_TEMPORAL),
+    dict(kernel=golhip.KERNEL_TEMPORAL, temporal_steps=1),
+    dict(kernel=golhip.KERNEL_TEMPORAL, temporal_steps=5),
+]
+SHAPES = [(1, 16, 8), (1, 32, 16), (2, 16, 8), (4, 8, 8), (4, 8, 16), (8, 4, 16), (8, 8, 8)]
+
+
+def note(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def run_gpu(words, x, y, gens, word_bits=64, **kw):
+    with golhip.Grid(x, y, word_bits=word_bits, **kw) as g:
+        g.upload(words)
+        g.step(gens)
+        return g.download()
+
+
+def cells_for(f):
+    if f["generator"] == "random":
+        return ol.random_cells(f["x"], f["y"], f["seed"])
+    return ol.bernoulli_cells(f["x"], f["y"], f["seed"], f["p"])
+
+
+@pytest.mark.parametrize("kw", KERNELS, ids=lambda k: "-".join(f"{a}{b}" for a, b in k.items()))
+def test_naive_fixtures(kw):
+    for f in FIX["naive_runs"]:
+        note("fixture", f["x"], f["y"], f["generations"], kw)
+        words = ol.pack(cells_for(f), f["word_bits"])
+        out = run_gpu(words, f["x"], f["y"], f["generations"], f["word_bits"], **kw)
+        assert ol.sha256(out) == f["sha256"], (f, kw)
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"dw{s[0]}-br{s[1]}-nw{s[2]}")
+@pytest.mark.parametrize("steps", [1, 3, 8, 16])
+def test_shapes_against_naive(shape, steps):
+    dw, br, nw = shape
+    for x, y, gens in [(64, 64, 37), (192, 150, 40), (2048 + 64, 33, 21), (32 * 64 * dw + 128, 70, 19),
+                       (128, 1, 9), (64, 2, 9), (4096, 257, 17)]:
+        if steps > 32 * dw and x // 32 > 64 * dw:
+            continue
+        if nw * br - 2 * steps < 1 and y > nw * br:
+            continue
+        note("shape", shape, "steps", steps, "grid", x, y, gens)
+        c = ol.bernoulli_cells(x, y, x + 7 * y + gens, 0.45)
+        ref = ol.naive_run(c, gens)
+        out = run_gpu(ol.pack(c), x, y, gens, kernel=golhip.KERNEL_TEMPORAL, temporal_steps=steps,
+                      lane_dwords=dw, band_rows=br, waves=nw)
+        got = ol.unpack(out, x, y)
+        if not np.array_equal(got, ref):
+            d = np.argwhere(got != ref)
+            pytest.fail(f"{len(d)} cells differ, first {d[:5].tolist()}")
+
+
+@pytest.mark.parametrize("dw", [1, 2, 4, 8])
+def test_plain_lane_widths(dw):
+    for x, y, gens in [(64, 64, 30), (320, 97, 25), (64 * 64 * dw + 64, 40, 12), (32, 5, 7)]:
+        note("plain dw", dw, x, y, gens)
+        wb = 64 if x % 64 == 0 else 32
+        c = ol.bernoulli_cells(x, y, 3 * x + y, 0.5)
+        ref = ol.naive_run(c, gens)
+        out = run_gpu(ol.pack(c, wb), x, y, gens, word_bits=wb, kernel=golhip.KERNEL_PLAIN, lane_dwords=dw)
+        assert np.array_equal(ol.unpack(out, x, y, wb), ref)
+
+
+def test_every_generation_matches():
+    x, y = 256, 96
+    c = ol.random_cells(x, y, 5)
+    with golhip.Grid(x, y, kernel=golhip.KERNEL_TEMPORAL, temporal_steps=4) as g:
+        g.upload(ol.pack(c))
+        for i in range(80):
+            note("generation", i)
+            g.step(1)
+            c = ol.naive_run(c, 1)
+            assert np.array_equal(ol.unpack(g.download(), x, y), c), i
+
+
+def test_split_steps_equal_one_step():
+    x = y = 1024
+    w = golhip.fill_random(x, y, 11)
+    full = run_gpu(w, x, y, 123)
+    with golhip.Grid(x, y) as g:
+        g.upload(w)
+        for k in (1, 2, 3, 17, 100):
+            g.step(k)
+        assert np.array_equal(g.download(), full)
+
+
+def test_zero_generations_identity():
+    w = golhip.fill_random(512, 64, 3)
+    for kw in KERNELS:
+        assert np.array_equal(run_gpu(w, 512, 64, 0, **kw), w)
+
+
+@pytest.mark.parametrize("gens", [1, 100, 1000, 10000])
+def test_full_size_random_16384(gens):
+    f = [r for r in FIX["bitpar_runs"] if r["name"] == "random-16384" and r["generations"] == gens][0]
+    w = golhip.fill_random(16384, 16384, 42)
+    assert ol.sha256(w) == f["sha256_input"]
+    note("16384^2 random gens", gens)
+    out = run_gpu(w, 16384, 16384, gens)
+    assert ol.sha256(out) == f["sha256"]
+
+
+def test_full_size_plain_equals_temporal():
+    w = golhip.fill_random(16384, 16384, 42)
+    note("16384^2 plain vs temporal, 100 gens")
+    a = run_gpu(w, 16384, 16384, 100, kernel=golhip.KERNEL_PLAIN)
+    b = run_gpu(w, 16384, 16384, 100, kernel=golhip.KERNEL_TEMPORAL, temporal_steps=7)
+    assert np.array_equal(a, b)
+    f = [r for r in FIX["bitpar_runs"] if r["name"] == "random-16384" and r["generations"] == 100][0]
+    assert ol.sha256(a) == f["sha256"]
+
+
+@pytest.mark.parametrize("gens", [100, 20000])
+def test_full_size_patches(gens):
+    f = [r for r in FIX["bitpar_runs"] if r["name"] == "patches-16384" and r["generations"] == gens][0]
+    p = f["params"]
+    w = golhip.fill_patches(16384, 16384, 42, p["n_patches"], p["patch"], p["p"])
+    assert ol.sha256(w) == f["sha256_input"]
+    note("16384^2 patches gens", gens)
+    out = run_gpu(w, 16384, 16384, gens)
+    assert ol.sha256(out) == f["sha256"]
+
+
+def test_rectangular_4096x8192():
+    f = [r for r in FIX["bitpar_runs"] if r["name"] == "random-4096x8192"][0]
+    w = golhip.fill_bernoulli(4096, 8192, 7, 0.4)
+    note("4096x8192 bernoulli 500")
+    out = run_gpu(w, 4096, 8192, 500)
+    assert ol.sha256(out) == f["sha256"]
+
+
+def test_wide_grid_x_tiles():
+    # wider than one wavefront row segment: tiles overlap by a one-lane halo
+    x, y, gens = 16384 + 4096, 200, 50
+    c = ol.bernoulli_cells(x, y, 1, 0.5)
+    w = ol.pack(c)
+    ref = ol.unpack(ol.bitpar_run(w, x, y, gens), x, y)
+    for steps in (8, 25):
+        note("wide", x, y, steps)
+        out = run_gpu(w, x, y, gens, kernel=golhip.KERNEL_TEMPORAL, temporal_steps=steps)
+        assert np.array_equal(ol.unpack(out, x, y), ref)

@@ -1,0 +1,184 @@
+"""Pins the CPU oracle (oracle/gol_oracle.cpp) before anything is checked against it.
+
+Each check is chosen so that a plausible mistake in the restatement fails it:
+  * AN5D alive-count KATs (AN5D/GOL_AN5D.c:92-97) — an independent record of
+    the rule and the neighbour count (a dropped neighbour, a wrong birth or
+    survival count changes the counts);
+  * the B3/S23 table over all 512 3x3 neighbourhoods, centred and at every
+    grid edge and corner (pins the dead boundary: cells outside never count);
+  * still lifes, oscillators and glider translation (closed forms);
+  * the eight symmetries of the square (a transposed operand or a wrong
+    index fails these);
+  * the bit-parallel engine against the naive loop on ragged sizes;
+  * the loaders: mt19937's standard 10000th value and the seed-42 fixture.
+"""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+FIX = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "fixtures.json")))
+
+# The rule as a table: next state indexed by [alive][live neighbours].
+B3S23 = np.array([[0, 0, 0, 1, 0, 0, 0, 0, 0],
+                  [0, 0, 1, 1, 0, 0, 0, 0, 0]], dtype=np.uint8)
+
+
+def test_mt19937_standard_value():
+    # [rand.predef]: the 10000th invocation of a default-constructed mt19937 is 4123659995
+    assert ol.lib().oracle_mt19937_nth(5489, 10000) == FIX["mt19937_10000"] == 4123659995
+
+
+def test_loader_fixture():
+    f = FIX["loader"]
+    c = ol.random_cells(f["x"], f["y"], f["seed"])
+    assert "".join(str(v) for v in c.reshape(-1)[:256]) == f["first_256"]
+    assert ol.sha256(c) == f["sha256_cells"]
+    assert int(c.sum()) == f["popcount"]
+    assert set(np.unique(c)) <= {0, 1}
+
+
+@pytest.mark.parametrize("n", [256, 512])
+def test_an5d_kat(n):
+    g = ol.naive_run(ol.glibc_rand_cells(n, 1985), 1024, periodic=True)
+    assert int(g.sum()) == FIX["an5d_kats"][str(n)]
+
+
+@pytest.mark.slow
+def test_an5d_kat_1024():
+    g = ol.naive_run(ol.glibc_rand_cells(1024, 1985), 1024, periodic=True)
+    assert int(g.sum()) == FIX["an5d_kats"]["1024"]
+
+
+def test_all_neighbourhoods_centred():
+    # each 3x3 pattern in the middle of a 5x5 grid: the centre follows the table
+    for pat in range(512):
+        g = np.zeros((5, 5), np.uint8)
+        bits = [(pat >> k) & 1 for k in range(9)]
+        g[1:4, 1:4] = np.array(bits, np.uint8).reshape(3, 3)
+        out = ol.naive_run(g, 1)
+        alive = bits[4]
+        n = sum(bits) - alive
+        assert out[2, 2] == B3S23[alive, n], pat
+
+
+def test_all_neighbourhoods_on_a_3x3_grid():
+    # every cell of a 3x3 grid is on an edge: only in-grid cells count
+    for pat in range(512):
+        g = np.array([(pat >> k) & 1 for k in range(9)], np.uint8).reshape(3, 3)
+        out = ol.naive_run(g, 1)
+        for y in range(3):
+            for x in range(3):
+                n = int(g[max(0, y - 1):y + 2, max(0, x - 1):x + 2].sum()) - int(g[y, x])
+                assert out[y, x] == B3S23[g[y, x], n], (pat, y, x)
+
+
+def test_still_lifes_and_oscillators():
+    g = np.zeros((12, 12), np.uint8)
+    g[1:3, 1:3] = 1                                   # block
+    g[6, 7:9] = 1; g[7, 6] = 1; g[7, 9] = 1; g[8, 7:9] = 1  # beehive
+    assert np.array_equal(ol.naive_run(g, 1), g)
+    assert np.array_equal(ol.naive_run(g, 17), g)
+    b = np.zeros((5, 5), np.uint8)
+    b[2, 1:4] = 1                                     # blinker (patterns.cpp:1593)
+    b1 = ol.naive_run(b, 1)
+    assert np.array_equal(b1, b.T)
+    assert np.array_equal(ol.naive_run(b, 2), b)
+    # a blinker touching the grid edge still oscillates (its outside cells stay dead)
+    e = np.zeros((3, 3), np.uint8)
+    e[0, :] = 1
+    assert np.array_equal(ol.naive_run(e, 1), np.array([[0, 1, 0], [0, 1, 0], [0, 0, 0]], np.uint8))
+
+
+def test_glider_translates():
+    # the lexicon's glider (patterns.cpp:4237: XXX / X.. / .X.) moves (-1,-1) every 4 generations
+    g = np.zeros((20, 20), np.uint8)
+    g[10, 10:13] = 1
+    g[11, 10] = 1
+    g[12, 11] = 1
+    out = ol.naive_run(g, 4)
+    assert np.array_equal(out, np.roll(np.roll(g, -1, 0), -1, 1))
+    out = ol.naive_run(g, 32)
+    assert np.array_equal(out, np.roll(np.roll(g, -8, 0), -8, 1))
+
+
+def test_glider_dies_into_block_at_corner():
+    # dead boundary: a glider flying into the corner ends as a still life, never wraps
+    g = np.zeros((8, 8), np.uint8)
+    g[4, 4:7] = 1
+    g[5, 4] = 1
+    g[6, 5] = 1
+    out = ol.naive_run(g, 200)
+    assert np.array_equal(ol.naive_run(out, 1), out)
+    assert out[6:, :].sum() == 0 and out[:, 6:].sum() == 0
+
+
+@pytest.mark.parametrize("shape", [(37, 53), (64, 96), (1, 40), (40, 1), (2, 2)])
+def test_symmetries(shape):
+    rng = np.random.default_rng(sum(shape))
+    g = (rng.random(shape) < 0.4).astype(np.uint8)
+    ref = ol.naive_run(g, 9)
+    for f in (np.transpose, np.fliplr, np.flipud, lambda a: np.rot90(a, 1), lambda a: np.rot90(a, 2),
+              lambda a: np.rot90(a, 3), lambda a: np.rot90(a.T, 1)):
+        assert np.array_equal(ol.naive_run(np.ascontiguousarray(f(g)), 9), f(ref))
+
+
+def test_zero_generations_is_identity():
+    g = ol.random_cells(64, 8, 3)
+    assert np.array_equal(ol.naive_run(g, 0), g)
+    w = ol.pack(g)
+    assert np.array_equal(ol.bitpar_run(w, 64, 8, 0), w)
+
+
+@pytest.mark.parametrize("x,y,gens,p", [(64, 64, 40, 0.5), (128, 37, 25, 0.3), (192, 200, 60, 0.45),
+                                        (64, 1, 5, 0.6), (512, 96, 33, 0.5)])
+def test_bitpar_matches_naive(x, y, gens, p):
+    g = ol.bernoulli_cells(x, y, x * 1000 + y, p)
+    ref = ol.naive_run(g, gens)
+    got = ol.unpack(ol.bitpar_run(ol.pack(g), x, y, gens), x, y)
+    assert np.array_equal(got, ref)
+
+
+def test_bitpar_matches_naive_every_generation():
+    g = ol.random_cells(128, 64, 77)
+    w = ol.pack(g)
+    for _ in range(60):
+        g = ol.naive_run(g, 1)
+        w = ol.bitpar_run(w, 128, 64, 1)
+        assert np.array_equal(ol.unpack(w, 128, 64), g)
+
+
+def test_pack_places_bits():
+    g = np.zeros((3, 128), np.uint8)
+    g[1, 0] = 1
+    g[2, 127] = 1
+    g[0, 65] = 1
+    w = ol.pack(g, 64)
+    assert w.tolist() == [0, 2, 1, 0, 0, 1 << 63]
+    w32 = ol.pack(g, 32)
+    assert w32[4] == 1 and w32[11] == 1 << 31 and w32[2] == 2
+    # rows-64 and rows-32 are the same bytes on a little-endian host
+    assert w.tobytes() == w32.tobytes()
+    assert np.array_equal(ol.unpack(w, 128, 3, 64), g)
+    assert np.array_equal(ol.unpack(w32, 128, 3, 32), g)
+
+
+def test_pack_rejects_ragged_width():
+    with pytest.raises(ValueError):
+        ol.pack(np.zeros((2, 96), np.uint8), 64)
+
+
+def test_naive_fixtures_small():
+    # re-run the cheap fixture cases (the 2048^2 x 1000 case is covered by a slow test)
+    for f in FIX["naive_runs"]:
+        if f["x"] * f["y"] * f["generations"] > 5e7:
+            continue
+        g = (ol.random_cells(f["x"], f["y"], f["seed"]) if f["generator"] == "random"
+             else ol.bernoulli_cells(f["x"], f["y"], f["seed"], f["p"]))
+        out = ol.naive_run(g, f["generations"])
+        assert ol.sha256(ol.pack(out, f["word_bits"])) == f["sha256"], f
+        assert int(out.sum()) == f["popcount"]
