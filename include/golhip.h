@@ -54,7 +54,9 @@ enum gol_kernel {
                                 (reference: cuda-naive-bitwise/cuda_naive_bitwise_kernel.cu:32-79) */
     GOL_KERNEL_TEMPORAL = 2, /* T generations per launch on LDS-exchanged register tiles
                                 (reference: cuda-temporal-rowed/temporal_rowed.cu:33-229) */
-    GOL_KERNEL_ACTIVE = 3    /* change tracking over a device-side active-tile list
+    GOL_KERNEL_ACTIVE = 3    /* change tracking over a device-side active-tile list: a tile is
+                                advanced only when it or a neighbour changed over the last block
+                                of temporal_steps generations
                                 (reference: cuda-naive-local-one-cell/cuda_local_one_cell.cu:53-242) */
 };
 
@@ -82,7 +84,8 @@ typedef struct gol_info {
     uint32_t tiles;           /* workgroups per launch */
     uint64_t launches;        /* kernel launches issued so far */
     uint64_t generations;     /* generations enqueued so far (actually_performed_iterations) */
-    uint64_t last_active_tiles; /* GOL_KERNEL_ACTIVE: tiles computed in the last generation */
+    uint64_t active_tiles_total; /* GOL_KERNEL_ACTIVE: tiles computed, summed over blocks (syncs) */
+    uint64_t tracked_blocks;     /* GOL_KERNEL_ACTIVE: blocks of temporal_steps generations run */
     uint64_t device_bytes;    /* device memory owned by the handle */
 } gol_info;
 
@@ -126,6 +129,45 @@ int gol_get_info(gol_handle* h, gol_info* info);
 
 /* Release the device state (reference: cudaFree in finalize_data_structures). */
 void gol_destroy(gol_handle* h);
+
+/* ---- Batches: many independent grids of uneven size, each with its own
+ * generation count (no reference counterpart: the reference runs one grid per
+ * process, src/infrastructure/experiment_manager.hpp:235-270). Each grid's
+ * result is bit-identical to running it alone through gol_step. ---- */
+typedef struct gol_batch gol_batch;
+
+typedef struct gol_batch_config {
+    int32_t device;             /* HIP device ordinal; -1 = current device */
+    uint32_t round_generations; /* generations per round for grids split into tiles (1..32); 0 = 32 */
+    uint32_t use_graph;         /* 1: record all rounds once into a hipGraph and replay it */
+    uint32_t reserved;
+} gol_batch_config;
+
+typedef struct gol_batch_info {
+    uint32_t grids;
+    uint32_t rounds;            /* rounds of work items */
+    uint64_t items;             /* work items (grid tile x round) */
+    uint64_t launches;          /* kernel launches per run (inside the graph when use_graph) */
+    uint64_t device_bytes;
+    uint64_t cell_updates;      /* sum over grids of cells x generations */
+} gol_batch_info;
+
+/* config may be NULL (current device, 32 generations per round, graph on). */
+int gol_batch_create(const gol_batch_config* config, gol_batch** out);
+/* Add a grid and its initial state (host words); returns its id. Only before
+ * the first gol_batch_run. */
+int gol_batch_add(gol_batch* b, uint64_t x_cells, uint64_t y_cells, uint32_t word_bits, uint64_t generations,
+                  const void* words, uint32_t* grid_id);
+/* Advance every grid by its generation count from its initial state and wait;
+ * gpu_ms (may be NULL) receives the GPU time of the run. Running again
+ * restarts from the initial states. */
+int gol_batch_run(gol_batch* b, double* gpu_ms);
+/* Restore every grid's initial state (done by gol_batch_run before a re-run). */
+int gol_batch_reset(gol_batch* b);
+/* Copy grid `grid_id`'s state out (its final state after a run). */
+int gol_batch_fetch(gol_batch* b, uint32_t grid_id, void* words, size_t n_words);
+int gol_batch_get_info(gol_batch* b, gol_batch_info* info);
+void gol_batch_destroy(gol_batch* b);
 
 /* Host-side input generators (no device work), writing packed row words:
  *  gol_fill_random: the reference's RandomOnesZerosDataLoader

@@ -1,0 +1,398 @@
+// Batched execution of many independent grids (no reference counterpart: the
+// reference runs one grid per process, src/infrastructure/experiment_manager.hpp:235-270).
+//
+// Every grid gets a tile shape from its size class (autotuner table, or a
+// measured choice from gol_batch_tune) and its own ping-pong buffers in one
+// device arena. Work is cut into rounds; in each round every unfinished grid
+// contributes one work item per tile, advancing that tile by the grid's
+// per-round generation count:
+//   * grids split into several tiles advance `round_generations` (the tile
+//     halo) per round, so neighbouring tiles exchange halos through the
+//     grid's buffers between rounds;
+//   * grids that fit one tile need no halo; they advance a chunk sized so the
+//     item costs about as much as a large tile's item, which keeps the items
+//     of a round balanced.
+// A round is one launch per tile shape present, one workgroup per item, items
+// ordered largest first. All rounds are recorded once into a hipGraph and
+// replayed, so a whole batch is one graph launch.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "../../include/golhip.h"
+#include "gol_capi_internal.hpp"
+#include "gol_kernels.hpp"
+
+using namespace golhip;
+
+namespace golhip {
+
+// Row budget of a round's work item, in tile rows x generations: a split
+// grid's item is (tile rows) x round_generations; one-tile grids get as many
+// generations per round as make the same product.
+static int chunk_generations(const TileShape& s, int round_generations, int big_tile_rows) {
+    const int rows = s.band_rows * s.waves;
+    const int64_t budget = static_cast<int64_t>(round_generations) * big_tile_rows;
+    return static_cast<int>(std::max<int64_t>(round_generations, budget / rows));
+}
+
+TileShape batch_default_shape(int64_t x_dw, int64_t y_rows) {
+    if (x_dw <= 64) {
+        if (y_rows <= 64) return TileShape{1, 8, 8};
+        if (y_rows <= 128) return TileShape{1, 16, 8};
+        return TileShape{1, 32, 16};
+    }
+    return default_shape(x_dw, y_rows);
+}
+
+// Size classes of the measured autotuner: (lane width class, row class).
+struct SizeClass {
+    int x_class, y_class;
+    bool operator<(const SizeClass& o) const { return x_class != o.x_class ? x_class < o.x_class : y_class < o.y_class; }
+};
+static SizeClass size_class(int64_t x_dw, int64_t y_rows) {
+    int xc = x_dw <= 64 ? 0 : (x_dw <= 128 ? 1 : (x_dw <= 256 ? 2 : 3));
+    int yc = y_rows <= 64 ? 0 : (y_rows <= 128 ? 1 : (y_rows <= 512 ? 2 : 3));
+    return SizeClass{xc, yc};
+}
+
+static std::mutex g_tune_mu;
+static std::map<SizeClass, TileShape> g_tuned;  // process-wide measured choices
+
+TileShape batch_shape(int64_t x_dw, int64_t y_rows) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(size_class(x_dw, y_rows));
+    if (it != g_tuned.end()) return it->second;
+    return batch_default_shape(x_dw, y_rows);
+}
+
+void set_tuned_shape(int64_t x_dw, int64_t y_rows, const TileShape& s) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[size_class(x_dw, y_rows)] = s;
+}
+
+}  // namespace golhip
+
+struct BatchEntry {
+    int64_t x_dw, y_rows;
+    uint32_t word_bits;
+    uint64_t generations;
+    std::vector<uint32_t> host;  // initial state until uploaded
+    size_t offset = 0;           // dword offset of buf0 in the arena (buf1, init follow)
+    TileShape shape{1, 16, 8};
+    TileGeometry geom{};
+    int chunk = 0;               // generations per round
+    int final_parity = 0;
+};
+
+struct gol_batch {
+    gol_batch_config cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<BatchEntry> grids;
+    bool built = false;
+    uint32_t* arena = nullptr;
+    size_t arena_dwords = 0;
+    BatchGrid* d_grids = nullptr;
+    WorkItem* d_items = nullptr;
+    struct Launch {
+        TileShape shape;
+        size_t first;  // index into the item array
+        int count;
+    };
+    std::vector<Launch> launches;
+    uint32_t rounds = 0;
+    uint64_t items = 0;
+    uint64_t cell_updates = 0;
+    uint64_t runs = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+};
+
+namespace {
+
+struct Guard {
+    int prev = -1;
+    explicit Guard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~Guard() {
+        int now = -1;
+        if (prev >= 0 && hipGetDevice(&now) == hipSuccess && now != prev) (void)hipSetDevice(prev);
+    }
+};
+
+// Plan shapes, rounds and items; allocate the arena and upload everything.
+int build(gol_batch* b) {
+    const int T = b->cfg.round_generations ? static_cast<int>(b->cfg.round_generations) : 32;
+    const int big_rows = 512;
+    size_t off = 0;
+    for (auto& g : b->grids) {
+        g.shape = batch_shape(g.x_dw, g.y_rows);
+        const bool one_tile = g.x_dw <= 64 * g.shape.lane_dwords && g.y_rows <= g.shape.band_rows * g.shape.waves;
+        const char* why = nullptr;
+        int per_round = one_tile ? chunk_generations(g.shape, T, big_rows) : T;
+        if (!plan_tiles(g.shape, g.x_dw, g.y_rows, one_tile ? 1 : per_round, &g.geom, &why))
+            return fail(GOL_ERR_RUNTIME, "batch grid %lldx%lld: %s", static_cast<long long>(g.x_dw * 32),
+                        static_cast<long long>(g.y_rows), why);
+        g.chunk = per_round;
+        g.offset = off;
+        const size_t n = static_cast<size_t>(g.x_dw * g.y_rows);
+        off += 3 * ((n + 63) & ~size_t(63));  // buf0, buf1, initial copy; 256-byte aligned
+    }
+    b->arena_dwords = off;
+    hipError_t e;
+    if (off > 0 && (e = hipMalloc(&b->arena, off * 4)) != hipSuccess) return fail_hip(e, "batch arena");
+
+    // descriptors
+    std::vector<BatchGrid> descs(b->grids.size());
+    for (size_t i = 0; i < b->grids.size(); ++i) {
+        auto& g = b->grids[i];
+        const size_t n = static_cast<size_t>(g.x_dw * g.y_rows);
+        const size_t stride = (n + 63) & ~size_t(63);
+        descs[i].buf[0] = b->arena + g.offset;
+        descs[i].buf[1] = b->arena + g.offset + stride;
+        descs[i].geo = to_device_geo(g.geom, g.x_dw, g.y_rows);
+        // initial copy and current state
+        if ((e = hipMemcpyAsync(b->arena + g.offset + 2 * stride, g.host.data(), n * 4, hipMemcpyHostToDevice,
+                                b->stream)) != hipSuccess ||
+            (e = hipMemcpyAsync(descs[i].buf[0], g.host.data(), n * 4, hipMemcpyHostToDevice, b->stream)) !=
+                hipSuccess)
+            return fail_hip(e, "batch upload");
+    }
+
+    // rounds of work items, grouped by shape, largest items first
+    std::vector<WorkItem> all;
+    std::vector<uint64_t> left(b->grids.size());
+    std::vector<int> parity(b->grids.size(), 0);
+    for (size_t i = 0; i < b->grids.size(); ++i) left[i] = b->grids[i].generations;
+    b->launches.clear();
+    b->rounds = 0;
+    b->cell_updates = 0;
+    for (;;) {
+        std::map<std::tuple<int, int, int>, std::vector<WorkItem>> by_shape;
+        bool any = false;
+        for (size_t i = 0; i < b->grids.size(); ++i) {
+            if (left[i] == 0) continue;
+            any = true;
+            auto& g = b->grids[i];
+            const int gens = static_cast<int>(std::min<uint64_t>(left[i], static_cast<uint64_t>(g.chunk)));
+            auto& v = by_shape[{g.shape.lane_dwords, g.shape.band_rows, g.shape.waves}];
+            const int tiles = g.geom.tiles_x * g.geom.tiles_y;
+            for (int t = 0; t < tiles; ++t) v.push_back(WorkItem{static_cast<int>(i), t, gens, parity[i]});
+            parity[i] ^= 1;
+            left[i] -= static_cast<uint64_t>(gens);
+            b->cell_updates += static_cast<uint64_t>(g.x_dw * 32 * g.y_rows) * static_cast<uint64_t>(gens);
+        }
+        if (!any) break;
+        for (auto& kv : by_shape) {
+            auto& v = kv.second;
+            std::stable_sort(v.begin(), v.end(), [](const WorkItem& a, const WorkItem& c) { return a.gens > c.gens; });
+            gol_batch::Launch L;
+            L.shape = TileShape{std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first)};
+            L.first = all.size();
+            L.count = static_cast<int>(v.size());
+            all.insert(all.end(), v.begin(), v.end());
+            b->launches.push_back(L);
+        }
+        b->rounds += 1;
+    }
+    for (size_t i = 0; i < b->grids.size(); ++i) b->grids[i].final_parity = parity[i];
+    b->items = all.size();
+
+    if (!descs.empty()) {
+        if ((e = hipMalloc(&b->d_grids, descs.size() * sizeof(BatchGrid))) != hipSuccess)
+            return fail_hip(e, "batch descriptors");
+        if ((e = hipMemcpyAsync(b->d_grids, descs.data(), descs.size() * sizeof(BatchGrid), hipMemcpyHostToDevice,
+                                b->stream)) != hipSuccess)
+            return fail_hip(e, "batch descriptors upload");
+    }
+    if (!all.empty()) {
+        if ((e = hipMalloc(&b->d_items, all.size() * sizeof(WorkItem))) != hipSuccess)
+            return fail_hip(e, "batch items");
+        if ((e = hipMemcpyAsync(b->d_items, all.data(), all.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
+                                b->stream)) != hipSuccess)
+            return fail_hip(e, "batch items upload");
+    }
+    if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return fail_hip(e, "batch build");
+    for (auto& g : b->grids) {
+        g.host.clear();
+        g.host.shrink_to_fit();
+    }
+    b->built = true;
+    return GOL_OK;
+}
+
+int enqueue_rounds(gol_batch* b) {
+    for (const auto& L : b->launches) {
+        hipError_t e = launch_items(L.shape, b->d_grids, b->d_items + L.first, L.count, b->stream);
+        if (e != hipSuccess) return fail_hip(e, "item_kernel launch");
+    }
+    return GOL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gol_batch_create(const gol_batch_config* config, gol_batch** out) {
+    clear_error();
+    if (!out) return fail(GOL_ERR_INVALID_ARGUMENT, "out is null");
+    *out = nullptr;
+    gol_batch_config cfg{};
+    cfg.device = -1;
+    cfg.use_graph = 1;
+    if (config) cfg = *config;
+    if (cfg.round_generations > 32)
+        return fail(GOL_ERR_INVALID_ARGUMENT, "round_generations must be at most 32 (the one-lane x halo)");
+    int dev = cfg.device;
+    hipError_t e;
+    if (dev < 0 && (e = hipGetDevice(&dev)) != hipSuccess) return fail_hip(e, "hipGetDevice");
+    gol_batch* b = new gol_batch();
+    b->cfg = cfg;
+    b->device = dev;
+    Guard g(dev);
+    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess) {
+        int rc = fail_hip(e, "gol_batch_create");
+        gol_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return GOL_OK;
+}
+
+int gol_batch_add(gol_batch* b, uint64_t x_cells, uint64_t y_cells, uint32_t word_bits, uint64_t generations,
+                  const void* words, uint32_t* grid_id) {
+    clear_error();
+    if (!b || !words) return fail(GOL_ERR_INVALID_ARGUMENT, "null argument");
+    if (b->built) return fail(GOL_ERR_RUNTIME, "grids cannot be added after the first gol_batch_run");
+    if (word_bits != 32 && word_bits != 64) return fail(GOL_ERR_INVALID_ARGUMENT, "word_bits must be 32 or 64");
+    if (x_cells == 0 || y_cells == 0 || x_cells % word_bits != 0)
+        return fail(GOL_ERR_INVALID_ARGUMENT, "Grid dimensions X must be a multiple of %u", word_bits);
+    if (x_cells / 32 > (1u << 20) || y_cells > (1u << 20) || generations > (1ull << 40))
+        return fail(GOL_ERR_INVALID_ARGUMENT, "grid or generation count too large for a batch");
+    BatchEntry g;
+    g.x_dw = static_cast<int64_t>(x_cells / 32);
+    g.y_rows = static_cast<int64_t>(y_cells);
+    g.word_bits = word_bits;
+    g.generations = generations;
+    const size_t n = static_cast<size_t>(g.x_dw * g.y_rows);
+    g.host.resize(n);
+    std::memcpy(g.host.data(), words, n * 4);
+    b->grids.push_back(std::move(g));
+    if (grid_id) *grid_id = static_cast<uint32_t>(b->grids.size() - 1);
+    return GOL_OK;
+}
+
+int gol_batch_reset(gol_batch* b) {
+    clear_error();
+    if (!b) return fail(GOL_ERR_INVALID_ARGUMENT, "null batch");
+    if (!b->built) return GOL_OK;
+    Guard g(b->device);
+    for (auto& e : b->grids) {
+        const size_t n = static_cast<size_t>(e.x_dw * e.y_rows);
+        const size_t stride = (n + 63) & ~size_t(63);
+        hipError_t r = hipMemcpyAsync(b->arena + e.offset, b->arena + e.offset + 2 * stride, n * 4,
+                                      hipMemcpyDeviceToDevice, b->stream);
+        if (r != hipSuccess) return fail_hip(r, "gol_batch_reset");
+    }
+    return GOL_OK;
+}
+
+int gol_batch_run(gol_batch* b, double* gpu_ms) {
+    clear_error();
+    if (!b) return fail(GOL_ERR_INVALID_ARGUMENT, "null batch");
+    Guard g(b->device);
+    int rc;
+    if (!b->built && (rc = build(b))) return rc;
+    if (b->runs > 0 && (rc = gol_batch_reset(b))) return rc;
+    hipError_t e = hipEventRecord(b->ev0, b->stream);
+    if (e != hipSuccess) return fail_hip(e, "hipEventRecord");
+    if (b->cfg.use_graph) {
+        if (!b->graph_exec) {
+            if ((e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+                return fail_hip(e, "hipStreamBeginCapture");
+            rc = enqueue_rounds(b);
+            hipGraph_t graph = nullptr;
+            e = hipStreamEndCapture(b->stream, &graph);
+            if (rc) return rc;
+            if (e != hipSuccess) return fail_hip(e, "hipStreamEndCapture");
+            b->graph = graph;
+            if ((e = hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0)) != hipSuccess)
+                return fail_hip(e, "hipGraphInstantiate");
+        }
+        if ((e = hipGraphLaunch(b->graph_exec, b->stream)) != hipSuccess) return fail_hip(e, "hipGraphLaunch");
+    } else if ((rc = enqueue_rounds(b))) {
+        return rc;
+    }
+    if ((e = hipEventRecord(b->ev1, b->stream)) != hipSuccess) return fail_hip(e, "hipEventRecord");
+    if ((e = hipEventSynchronize(b->ev1)) != hipSuccess) return fail_hip(e, "gol_batch_run");
+    if (gpu_ms) {
+        float ms = 0.f;
+        if ((e = hipEventElapsedTime(&ms, b->ev0, b->ev1)) != hipSuccess) return fail_hip(e, "hipEventElapsedTime");
+        *gpu_ms = ms;
+    }
+    b->runs += 1;
+    return GOL_OK;
+}
+
+int gol_batch_fetch(gol_batch* b, uint32_t grid_id, void* words, size_t n_words) {
+    clear_error();
+    if (!b || !words) return fail(GOL_ERR_INVALID_ARGUMENT, "null argument");
+    if (grid_id >= b->grids.size()) return fail(GOL_ERR_NOT_FOUND, "no grid %u in the batch", grid_id);
+    auto& g = b->grids[grid_id];
+    const size_t n = static_cast<size_t>(g.x_dw * g.y_rows);
+    if (n_words != n * 32 / g.word_bits)
+        return fail(GOL_ERR_INVALID_ARGUMENT, "expected %zu words, got %zu", n * 32 / g.word_bits, n_words);
+    if (!b->built || b->runs == 0) {
+        std::memcpy(words, g.host.data(), n * 4);  // not run yet: the initial state
+        return GOL_OK;
+    }
+    Guard gd(b->device);
+    const size_t stride = (n + 63) & ~size_t(63);
+    const uint32_t* src = b->arena + g.offset + (g.final_parity ? stride : 0);
+    hipError_t e = hipMemcpyAsync(words, src, n * 4, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return fail_hip(e, "gol_batch_fetch");
+    return GOL_OK;
+}
+
+int gol_batch_get_info(gol_batch* b, gol_batch_info* info) {
+    clear_error();
+    if (!b || !info) return fail(GOL_ERR_INVALID_ARGUMENT, "null argument");
+    std::memset(info, 0, sizeof(*info));
+    info->grids = static_cast<uint32_t>(b->grids.size());
+    info->rounds = b->rounds;
+    info->items = b->items;
+    info->launches = b->launches.size();
+    info->device_bytes = b->arena_dwords * 4 + b->items * sizeof(WorkItem) + b->grids.size() * sizeof(BatchGrid);
+    info->cell_updates = b->cell_updates;
+    return GOL_OK;
+}
+
+void gol_batch_destroy(gol_batch* b) {
+    if (!b) return;
+    {
+        Guard g(b->device);
+        if (b->stream) (void)hipStreamSynchronize(b->stream);
+        if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        if (b->arena) (void)hipFree(b->arena);
+        if (b->d_grids) (void)hipFree(b->d_grids);
+        if (b->d_items) (void)hipFree(b->d_items);
+        if (b->ev0) (void)hipEventDestroy(b->ev0);
+        if (b->ev1) (void)hipEventDestroy(b->ev1);
+        if (b->stream) (void)hipStreamDestroy(b->stream);
+    }
+    delete b;
+}
+
+}  // extern "C"

@@ -78,6 +78,14 @@ struct gol_handle {
     TileGeometry geom{};
     uint64_t launches = 0;
     uint64_t generations = 0;
+    // change tracking (GOL_KERNEL_ACTIVE)
+    uint32_t* changed = nullptr;  // [2][n_tiles] flags: tile interior changed over a block
+    int* lists = nullptr;         // [2][n_tiles] active-tile lists
+    int* counts = nullptr;        // [2] list lengths
+    unsigned long long* active_total = nullptr;
+    int n_tiles = 0;
+    uint64_t blocks = 0;          // change-tracking blocks run
+    bool flags_valid = false;     // false: the next block treats every tile as active
 };
 
 namespace {
@@ -103,15 +111,15 @@ int validate_config(const gol_config* c) {
 int plan(gol_handle* h) {
     const gol_config& c = h->cfg;
     h->kernel = c.kernel == GOL_KERNEL_AUTO ? GOL_KERNEL_TEMPORAL : c.kernel;
-    if (h->kernel == GOL_KERNEL_ACTIVE)
-        return fail(GOL_ERR_INVALID_ARGUMENT, "GOL_KERNEL_ACTIVE is not available in this build");
     if (h->kernel == GOL_KERNEL_PLAIN) {
         uint32_t dw = c.lane_dwords;
         if (dw == 0) dw = h->x_dw <= 64 ? 1 : (h->x_dw <= 128 ? 2 : 4);
         h->plain_dw = dw;
         return GOL_OK;
     }
-    TileShape s = default_shape(h->x_dw, h->y_rows);
+    // Change tracking wants narrow tiles (one dword per lane: 2048-cell-wide
+    // tiles) so that quiet regions of a wide grid can be skipped.
+    TileShape s = h->kernel == GOL_KERNEL_ACTIVE ? TileShape{1, 16, 8} : default_shape(h->x_dw, h->y_rows);
     if (c.lane_dwords) s.lane_dwords = static_cast<int>(c.lane_dwords);
     if (c.band_rows) s.band_rows = static_cast<int>(c.band_rows);
     if (c.waves) s.waves = static_cast<int>(c.waves);
@@ -121,17 +129,64 @@ int plan(gol_handle* h) {
     h->shape = s;
     h->one_tile = h->x_dw <= 64 * s.lane_dwords && h->y_rows <= s.band_rows * s.waves;
     int steps = static_cast<int>(c.temporal_steps);
-    if (steps == 0) steps = h->one_tile ? 1024 : 8;
+    // 12 generations per block: oscillators of period 1, 2, 3, 4, 6 and 12
+    // return to their state at block boundaries and count as unchanged.
+    if (steps == 0) steps = h->kernel == GOL_KERNEL_ACTIVE ? 12 : (h->one_tile ? 1024 : 8);
+    if (h->kernel == GOL_KERNEL_ACTIVE) h->one_tile = false;  // blocks keep their length
     if (steps < 1 || steps > 1 << 20) return fail(GOL_ERR_INVALID_ARGUMENT, "temporal_steps out of range");
     const char* why = nullptr;
     if (!plan_tiles(s, h->x_dw, h->y_rows, h->one_tile ? 1 : steps, &h->geom, &why))
         return fail(GOL_ERR_RUNTIME, "temporal_steps %d does not fit the tile: %s", steps, why);
     h->steps = steps;
+    h->n_tiles = h->geom.tiles_x * h->geom.tiles_y;
+    return GOL_OK;
+}
+
+// One change-tracking block of h->steps generations:
+//   build the active list from the last block's flags (every tile when the
+//   flags are not valid), then advance only the listed tiles. A skipped tile
+//   keeps the state it had two blocks ago in the output buffer, which equals
+//   its current state: it and its neighbours did not change over the last
+//   block, so the block starting now reproduces the same state (the tiles are
+//   at least `steps` cells wide and tall, the reach of `steps` generations).
+int enqueue_active_block(gol_handle* h) {
+    const int k = static_cast<int>(h->blocks & 1);
+    hipError_t e;
+    if (!h->flags_valid) {
+        // the previous-block flags say "changed" everywhere
+        if ((e = hipMemsetAsync(h->changed + (k ^ 1) * h->n_tiles, 0xff, h->n_tiles * sizeof(uint32_t), h->stream)) !=
+            hipSuccess)
+            return fail_hip(e, "hipMemsetAsync");
+        if ((e = hipMemsetAsync(h->counts + k, 0, sizeof(int), h->stream)) != hipSuccess)
+            return fail_hip(e, "hipMemsetAsync");
+        h->flags_valid = true;
+    }
+    e = launch_build_active_list(h->changed + (k ^ 1) * h->n_tiles, h->changed + k * h->n_tiles, h->geom.tiles_x,
+                                 h->geom.tiles_y, h->lists + k * h->n_tiles, h->counts + k, h->counts + (k ^ 1),
+                                 h->active_total, h->stream);
+    if (e != hipSuccess) return fail_hip(e, "build_active_list launch");
+    e = launch_active_tiles(h->geom, h->buf[h->cur], h->buf[h->cur ^ 1], h->x_dw, h->y_rows, h->steps,
+                            h->lists + k * h->n_tiles, h->counts + k, h->changed + k * h->n_tiles, h->stream);
+    if (e != hipSuccess) return fail_hip(e, "active_tile_kernel launch");
+    h->cur ^= 1;
+    h->launches += 2;
+    h->blocks += 1;
+    h->generations += static_cast<uint64_t>(h->steps);
     return GOL_OK;
 }
 
 int enqueue(gol_handle* h, uint64_t generations) {
     uint64_t left = generations;
+    if (h->kernel == GOL_KERNEL_ACTIVE) {
+        while (left >= static_cast<uint64_t>(h->steps)) {
+            int rc = enqueue_active_block(h);
+            if (rc) return rc;
+            left -= static_cast<uint64_t>(h->steps);
+        }
+        // a shorter block runs every tile: an oscillator that returns to its
+        // state after `steps` generations need not after fewer
+        if (left > 0) h->flags_valid = false;
+    }
     while (left > 0) {
         uint32_t* src = h->buf[h->cur];
         uint32_t* dst = h->buf[h->cur ^ 1];
@@ -201,8 +256,25 @@ int gol_create(const gol_config* config, gol_handle** out) {
     if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreate(&h->ev_start)) != hipSuccess || (e = hipEventCreate(&h->ev_stop)) != hipSuccess ||
         (e = hipMalloc(&h->buf[0], bytes)) != hipSuccess || (e = hipMalloc(&h->buf[1], bytes)) != hipSuccess ||
-        (e = hipMemsetAsync(h->buf[0], 0, bytes, h->stream)) != hipSuccess ||
-        (e = hipStreamSynchronize(h->stream)) != hipSuccess) {
+        (e = hipMemsetAsync(h->buf[0], 0, bytes, h->stream)) != hipSuccess) {
+        rc = fail_hip(e, "gol_create");
+        gol_destroy(h);
+        return rc;
+    }
+    if (h->kernel == GOL_KERNEL_ACTIVE) {
+        const size_t nt = static_cast<size_t>(h->n_tiles);
+        if ((e = hipMalloc(&h->changed, 2 * nt * sizeof(uint32_t))) != hipSuccess ||
+            (e = hipMalloc(&h->lists, 2 * nt * sizeof(int))) != hipSuccess ||
+            (e = hipMalloc(&h->counts, 2 * sizeof(int))) != hipSuccess ||
+            (e = hipMalloc(&h->active_total, sizeof(unsigned long long))) != hipSuccess ||
+            (e = hipMemsetAsync(h->counts, 0, 2 * sizeof(int), h->stream)) != hipSuccess ||
+            (e = hipMemsetAsync(h->active_total, 0, sizeof(unsigned long long), h->stream)) != hipSuccess) {
+            rc = fail_hip(e, "gol_create (change tracking state)");
+            gol_destroy(h);
+            return rc;
+        }
+    }
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) {
         rc = fail_hip(e, "gol_create");
         gol_destroy(h);
         return rc;
@@ -220,6 +292,7 @@ int gol_upload(gol_handle* h, const void* words, size_t n_words) {
     hipError_t e = hipMemcpyAsync(h->buf[h->cur], words, h->n_dwords * 4, hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return fail_hip(e, "gol_upload");
+    h->flags_valid = false;  // a new state: every tile is active in the next block
     return GOL_OK;
 }
 
@@ -231,6 +304,7 @@ int gol_upload_device(gol_handle* h, const void* device_words, size_t n_words) {
     DeviceGuard g(h->device);
     hipError_t e = hipMemcpyAsync(h->buf[h->cur], device_words, h->n_dwords * 4, hipMemcpyDeviceToDevice, h->stream);
     if (e != hipSuccess) return fail_hip(e, "gol_upload_device");
+    h->flags_valid = false;
     return GOL_OK;
 }
 
@@ -311,6 +385,16 @@ int gol_get_info(gol_handle* h, gol_info* info) {
     info->launches = h->launches;
     info->generations = h->generations;
     info->device_bytes = 2 * h->n_dwords * 4;
+    if (h->kernel == GOL_KERNEL_ACTIVE) {
+        DeviceGuard g(h->device);
+        unsigned long long total = 0;
+        hipError_t e = hipMemcpyAsync(&total, h->active_total, sizeof(total), hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) return fail_hip(e, "gol_get_info");
+        info->active_tiles_total = total;
+        info->tracked_blocks = h->blocks;
+        info->device_bytes += static_cast<uint64_t>(h->n_tiles) * 2 * (sizeof(uint32_t) + sizeof(int)) + 16;
+    }
     return GOL_OK;
 }
 
@@ -321,6 +405,10 @@ void gol_destroy(gol_handle* h) {
         if (h->stream) (void)hipStreamSynchronize(h->stream);
         if (h->buf[0]) (void)hipFree(h->buf[0]);
         if (h->buf[1]) (void)hipFree(h->buf[1]);
+        if (h->changed) (void)hipFree(h->changed);
+        if (h->lists) (void)hipFree(h->lists);
+        if (h->counts) (void)hipFree(h->counts);
+        if (h->active_total) (void)hipFree(h->active_total);
         if (h->ev_start) (void)hipEventDestroy(h->ev_start);
         if (h->ev_stop) (void)hipEventDestroy(h->ev_stop);
         if (h->stream) (void)hipStreamDestroy(h->stream);

@@ -15,6 +15,11 @@ void clear_error();
 const char* last_error();
 
 TileShape default_shape(int64_t x_dw, int64_t y_rows);
+
+// Batch size classes: the measured choice if gol_batch_tune found one, else the table.
+TileShape batch_default_shape(int64_t x_dw, int64_t y_rows);
+TileShape batch_shape(int64_t x_dw, int64_t y_rows);
+void set_tuned_shape(int64_t x_dw, int64_t y_rows, const TileShape& s);
 bool shape_compiled(const TileShape& s);
 
 }  // namespace golhip

@@ -144,13 +144,10 @@ __global__ __launch_bounds__(64 * PLAIN_WAVES) void plain_kernel(const uint32_t*
 }
 
 // ---------------------------------------------------------------------------
-// Temporal tile kernel.
+// Temporal tile: one workgroup advances one tile of one grid by `gens`
+// generations and writes the tile's interior. Shared by the single-grid
+// launch, the active-tile launch and the batched work-item launch.
 // ---------------------------------------------------------------------------
-struct TileArgs {
-    int x_dw, y_rows, gens;
-    int halo_rows, halo_lanes;
-    int stride_rows, stride_dwords, tiles_x;
-};
 
 template <int DW, int BR, int NW, bool MASK_ROWS, bool MASK_COLS>
 __device__ __forceinline__ void run_generations(uint32_t (&s)[BR][DW], uint32_t (*slots)[NW][2][64 * DW], int gens,
@@ -174,14 +171,16 @@ __device__ __forceinline__ void run_generations(uint32_t (&s)[BR][DW], uint32_t 
     }
 }
 
-template <int DW, int BR, int NW>
-__global__ __launch_bounds__(64 * NW) void tile_kernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
-                                                       TileArgs a) {
-    __shared__ uint32_t slots[2][NW][2][64 * DW];
+// Returns (in every lane) whether any stored interior dword differs from src
+// when TRACK is set.
+template <int DW, int BR, int NW, bool TRACK>
+__device__ __forceinline__ bool process_tile(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                             const TileGeo& a, int tile, int gens,
+                                             uint32_t (*slots)[NW][2][64 * DW]) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-    const int tx = blockIdx.x % a.tiles_x;
-    const int ty = blockIdx.x / a.tiles_x;
+    const int tx = tile % a.tiles_x;
+    const int ty = tile / a.tiles_x;
     const int x0 = tx * a.stride_dwords - a.halo_lanes * DW;
     const int y0 = ty * a.stride_rows - a.halo_rows;
     const int gx0 = x0 + lane * DW;
@@ -202,47 +201,109 @@ __global__ __launch_bounds__(64 * NW) void tile_kernel(const uint32_t* __restric
     const bool edge_y = y0 < 0 || y0 + NW * BR > a.y_rows;  // workgroup-uniform
     const bool edge_x = x0 < 0 || x0 + 64 * DW > a.x_dw;
     if (edge_x) {
-        run_generations<DW, BR, NW, true, true>(s, slots, a.gens, lane, w, row_lo, row_hi, colmask);
+        run_generations<DW, BR, NW, true, true>(s, slots, gens, lane, w, row_lo, row_hi, colmask);
     } else if (edge_y) {
-        run_generations<DW, BR, NW, true, false>(s, slots, a.gens, lane, w, row_lo, row_hi, colmask);
+        run_generations<DW, BR, NW, true, false>(s, slots, gens, lane, w, row_lo, row_hi, colmask);
     } else {
-        run_generations<DW, BR, NW, false, false>(s, slots, a.gens, lane, w, row_lo, row_hi, colmask);
+        run_generations<DW, BR, NW, false, false>(s, slots, gens, lane, w, row_lo, row_hi, colmask);
     }
 
     const int tile_rows = NW * BR;
     const bool lane_inside = lane >= a.halo_lanes && lane < 64 - a.halo_lanes;
+    bool changed = false;
 #pragma unroll
     for (int r = 0; r < BR; ++r) {
         const int tr = w * BR + r;
         const int gy = gy0 + r;
         if (tr >= a.halo_rows && tr < tile_rows - a.halo_rows && gy >= 0 && gy < a.y_rows && lane_inside) {
+            if (TRACK) {
+                uint32_t before[DW];
+                load_seg<DW>(src, a.x_dw, a.y_rows, gy, gx0, vec_ok, before);
+#pragma unroll
+                for (int d = 0; d < DW; ++d) changed |= (before[d] != s[r][d]);
+            }
             store_seg<DW>(dst, a.x_dw, gy, gx0, vec_ok, s[r]);
         }
     }
+    return changed;
 }
 
 template <int DW, int BR, int NW>
-hipError_t launch_tile_shape(const TileGeometry& g, const uint32_t* src, uint32_t* dst, int x_dw, int y_rows, int gens,
-                             hipStream_t stream) {
-    TileArgs a;
-    a.x_dw = x_dw;
-    a.y_rows = y_rows;
-    a.gens = gens;
-    a.halo_rows = g.halo_rows;
-    a.halo_lanes = g.halo_lanes;
-    a.stride_rows = g.stride_rows;
-    a.stride_dwords = g.stride_dwords;
-    a.tiles_x = g.tiles_x;
-    const unsigned blocks = static_cast<unsigned>(g.tiles_x) * static_cast<unsigned>(g.tiles_y);
-    hipLaunchKernelGGL((tile_kernel<DW, BR, NW>), dim3(blocks), dim3(64 * NW), 0, stream, src, dst, a);
-    return hipGetLastError();
+__global__ __launch_bounds__(64 * NW) void tile_kernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                                       TileGeo a, int gens) {
+    __shared__ uint32_t slots[2][NW][2][64 * DW];
+    process_tile<DW, BR, NW, false>(src, dst, a, blockIdx.x, gens, slots);
+}
+
+// Change tracking: workgroup i advances tile list[i] (i < *count) and raises
+// changed[tile] when its interior differs after the block of generations.
+template <int DW, int BR, int NW>
+__global__ __launch_bounds__(64 * NW) void active_tile_kernel(const uint32_t* __restrict__ src,
+                                                              uint32_t* __restrict__ dst, TileGeo a, int gens,
+                                                              const int* __restrict__ list,
+                                                              const int* __restrict__ count,
+                                                              uint32_t* __restrict__ changed) {
+    __shared__ uint32_t slots[2][NW][2][64 * DW];
+    if (static_cast<int>(blockIdx.x) >= *count) return;  // whole workgroup leaves before any barrier
+    const int tile = list[blockIdx.x];
+    const bool c = process_tile<DW, BR, NW, true>(src, dst, a, tile, gens, slots);
+    if (__ballot(c) != 0 && (threadIdx.x & 63) == 0) changed[tile] = 1u;
+}
+
+// Batched grids: workgroup i advances work item i (a tile of one grid).
+template <int DW, int BR, int NW>
+__global__ __launch_bounds__(64 * NW) void item_kernel(const BatchGrid* __restrict__ grids,
+                                                       const WorkItem* __restrict__ items) {
+    __shared__ uint32_t slots[2][NW][2][64 * DW];
+    const WorkItem it = items[blockIdx.x];
+    const BatchGrid& g = grids[it.grid];
+    const uint32_t* src = g.buf[it.parity];
+    uint32_t* dst = g.buf[it.parity ^ 1];
+    process_tile<DW, BR, NW, false>(src, dst, g.geo, it.tile, it.gens, slots);
+}
+
+// Active-tile list for the next block: tile t is active when it or one of its
+// eight neighbours changed in the last block. Also clears the flags the next
+// block writes and the counter of the list after next.
+__global__ void build_active_list(const uint32_t* __restrict__ changed_prev, uint32_t* __restrict__ changed_next,
+                                  int tiles_x, int tiles_y, int* __restrict__ list, int* __restrict__ count,
+                                  int* __restrict__ count_other, unsigned long long* __restrict__ total) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) *count_other = 0;
+    if (t >= tiles_x * tiles_y) return;
+    const int tx = t % tiles_x, ty = t / tiles_x;
+    bool act = false;
+    for (int dy = -1; dy <= 1; ++dy) {
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int nx = tx + dx, ny = ty + dy;
+            if (nx >= 0 && nx < tiles_x && ny >= 0 && ny < tiles_y) act |= changed_prev[ny * tiles_x + nx] != 0;
+        }
+    }
+    changed_next[t] = 0;
+    if (act) {
+        const int i = atomicAdd(count, 1);
+        list[i] = t;
+        atomicAdd(total, 1ull);
+    }
 }
 
 }  // namespace
 
 const TileShape kTileShapes[] = {
-    {8, 8, 8}, {8, 4, 16}, {4, 8, 8}, {4, 8, 16}, {2, 16, 8}, {1, 16, 8}, {1, 32, 16},
+    {8, 8, 8}, {8, 4, 16}, {4, 8, 8}, {4, 8, 16}, {2, 16, 8}, {1, 8, 8}, {1, 16, 8}, {1, 32, 16},
 };
+
+TileGeo to_device_geo(const TileGeometry& g, int64_t x_dw, int64_t y_rows) {
+    TileGeo a;
+    a.x_dw = static_cast<int>(x_dw);
+    a.y_rows = static_cast<int>(y_rows);
+    a.halo_rows = g.halo_rows;
+    a.halo_lanes = g.halo_lanes;
+    a.stride_rows = g.stride_rows;
+    a.stride_dwords = g.stride_dwords;
+    a.tiles_x = g.tiles_x;
+    return a;
+}
 const int kNumTileShapes = static_cast<int>(sizeof(kTileShapes) / sizeof(kTileShapes[0]));
 
 bool plan_tiles(const TileShape& shape, int64_t x_dw, int64_t y_rows, int max_gens, TileGeometry* g,
@@ -282,20 +343,65 @@ bool plan_tiles(const TileShape& shape, int64_t x_dw, int64_t y_rows, int max_ge
     return true;
 }
 
+#define GOLHIP_FOR_EACH_SHAPE(X) \
+    X(8, 8, 8)                     \
+    X(8, 4, 16)                    \
+    X(4, 8, 8)                     \
+    X(4, 8, 16)                    \
+    X(2, 16, 8)                    \
+    X(1, 8, 8)                     \
+    X(1, 16, 8)                    \
+    X(1, 32, 16)
+
 hipError_t launch_tiles(const TileGeometry& g, const uint32_t* src, uint32_t* dst, int64_t x_dw, int64_t y_rows,
                         int gens, hipStream_t stream) {
-    const int xd = static_cast<int>(x_dw), yr = static_cast<int>(y_rows);
-#define GOLHIP_SHAPE(DW, BR, NW)                                                      \
-    if (g.lane_dwords == DW && g.band_rows == BR && g.waves == NW)                    \
-        return launch_tile_shape<DW, BR, NW>(g, src, dst, xd, yr, gens, stream);
-    GOLHIP_SHAPE(8, 8, 8)
-    GOLHIP_SHAPE(8, 4, 16)
-    GOLHIP_SHAPE(4, 8, 8)
-    GOLHIP_SHAPE(4, 8, 16)
-    GOLHIP_SHAPE(2, 16, 8)
-    GOLHIP_SHAPE(1, 16, 8)
-    GOLHIP_SHAPE(1, 32, 16)
-#undef GOLHIP_SHAPE
+    const unsigned blocks = static_cast<unsigned>(g.tiles_x) * static_cast<unsigned>(g.tiles_y);
+#define GOLHIP_LAUNCH(DW, BR, NW)                                                                        \
+    if (g.lane_dwords == DW && g.band_rows == BR && g.waves == NW) {                                     \
+        const TileGeo a = to_device_geo(g, x_dw, y_rows);                                                \
+        hipLaunchKernelGGL((tile_kernel<DW, BR, NW>), dim3(blocks), dim3(64 * NW), 0, stream, src, dst, a, gens); \
+        return hipGetLastError();                                                                        \
+    }
+    GOLHIP_FOR_EACH_SHAPE(GOLHIP_LAUNCH)
+#undef GOLHIP_LAUNCH
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_active_tiles(const TileGeometry& g, const uint32_t* src, uint32_t* dst, int64_t x_dw,
+                               int64_t y_rows, int gens, const int* list, const int* count, uint32_t* changed,
+                               hipStream_t stream) {
+    const unsigned blocks = static_cast<unsigned>(g.tiles_x) * static_cast<unsigned>(g.tiles_y);
+#define GOLHIP_LAUNCH(DW, BR, NW)                                                                        \
+    if (g.lane_dwords == DW && g.band_rows == BR && g.waves == NW) {                                     \
+        const TileGeo a = to_device_geo(g, x_dw, y_rows);                                                \
+        hipLaunchKernelGGL((active_tile_kernel<DW, BR, NW>), dim3(blocks), dim3(64 * NW), 0, stream, src, dst, \
+                           a, gens, list, count, changed);                                               \
+        return hipGetLastError();                                                                        \
+    }
+    GOLHIP_FOR_EACH_SHAPE(GOLHIP_LAUNCH)
+#undef GOLHIP_LAUNCH
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_build_active_list(const uint32_t* changed_prev, uint32_t* changed_next, int tiles_x, int tiles_y,
+                                    int* list, int* count, int* count_other, unsigned long long* total,
+                                    hipStream_t stream) {
+    const int n = tiles_x * tiles_y;
+    hipLaunchKernelGGL(build_active_list, dim3((n + 255) / 256), dim3(256), 0, stream, changed_prev, changed_next,
+                       tiles_x, tiles_y, list, count, count_other, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_items(const TileShape& shape, const BatchGrid* grids, const WorkItem* items, int n_items,
+                        hipStream_t stream) {
+    if (n_items <= 0) return hipSuccess;
+#define GOLHIP_LAUNCH(DW, BR, NW)                                                                        \
+    if (shape.lane_dwords == DW && shape.band_rows == BR && shape.waves == NW) {                         \
+        hipLaunchKernelGGL((item_kernel<DW, BR, NW>), dim3(n_items), dim3(64 * NW), 0, stream, grids, items); \
+        return hipGetLastError();                                                                        \
+    }
+    GOLHIP_FOR_EACH_SHAPE(GOLHIP_LAUNCH)
+#undef GOLHIP_LAUNCH
     return hipErrorInvalidValue;
 }
 

@@ -20,6 +20,28 @@ struct TileGeometry {
     int tiles_x, tiles_y;
 };
 
+// Device-side geometry of one grid under one tile shape.
+struct TileGeo {
+    int x_dw, y_rows;
+    int halo_rows, halo_lanes;
+    int stride_rows, stride_dwords, tiles_x;
+};
+
+// One grid of a batch: its ping-pong buffers and tile geometry.
+struct BatchGrid {
+    uint32_t* buf[2];
+    TileGeo geo;
+};
+
+// One unit of batched work: advance tile `tile` of grid `grid` by `gens`
+// generations from buf[parity] into buf[parity ^ 1].
+struct WorkItem {
+    int grid;
+    int tile;
+    int gens;
+    int parity;
+};
+
 // Tile kernel shapes compiled into the library: (DW, BR, NW).
 struct TileShape {
     int lane_dwords, band_rows, waves;
@@ -36,6 +58,24 @@ bool plan_tiles(const TileShape& shape, int64_t x_dw, int64_t y_rows, int max_ge
 // One launch: advance src by gens (<= the planned max_gens) generations into dst.
 hipError_t launch_tiles(const TileGeometry& g, const uint32_t* src, uint32_t* dst, int64_t x_dw, int64_t y_rows,
                         int gens, hipStream_t stream);
+
+TileGeo to_device_geo(const TileGeometry& g, int64_t x_dw, int64_t y_rows);
+
+// Change tracking: advance the tiles in list[0 .. *count) and raise
+// changed[tile] for tiles whose interior changed over the block.
+hipError_t launch_active_tiles(const TileGeometry& g, const uint32_t* src, uint32_t* dst, int64_t x_dw,
+                               int64_t y_rows, int gens, const int* list, const int* count, uint32_t* changed,
+                               hipStream_t stream);
+
+// Build the next active list from the last block's changed flags (tile or
+// any of its 8 neighbours changed); clears changed_next and *count_other.
+hipError_t launch_build_active_list(const uint32_t* changed_prev, uint32_t* changed_next, int tiles_x, int tiles_y,
+                                    int* list, int* count, int* count_other, unsigned long long* total,
+                                    hipStream_t stream);
+
+// Batched grids: one workgroup per work item, all items of one tile shape.
+hipError_t launch_items(const TileShape& shape, const BatchGrid* grids, const WorkItem* items, int n_items,
+                        hipStream_t stream);
 
 // Plain kernel: one generation per launch, global memory to global memory.
 // lane_dwords in {1,2,4,8}.

@@ -36,6 +36,8 @@ EXPORTED_SYMBOLS = (
     "gol_create", "gol_upload", "gol_upload_device", "gol_step", "gol_step_timed", "gol_download", "gol_sync",
     "gol_device_state", "gol_get_stream", "gol_get_info", "gol_destroy", "gol_last_error",
     "gol_version", "gol_fill_random", "gol_fill_bernoulli", "gol_fill_patches",
+    "gol_batch_create", "gol_batch_add", "gol_batch_run", "gol_batch_reset", "gol_batch_fetch",
+    "gol_batch_get_info", "gol_batch_destroy",
 )
 
 
@@ -70,8 +72,29 @@ class GolInfo(ctypes.Structure):
         ("tiles", ctypes.c_uint32),
         ("launches", ctypes.c_uint64),
         ("generations", ctypes.c_uint64),
-        ("last_active_tiles", ctypes.c_uint64),
+        ("active_tiles_total", ctypes.c_uint64),
+        ("tracked_blocks", ctypes.c_uint64),
         ("device_bytes", ctypes.c_uint64),
+    ]
+
+
+class GolBatchConfig(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int32),
+        ("round_generations", ctypes.c_uint32),
+        ("use_graph", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class GolBatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("grids", ctypes.c_uint32),
+        ("rounds", ctypes.c_uint32),
+        ("items", ctypes.c_uint64),
+        ("launches", ctypes.c_uint64),
+        ("device_bytes", ctypes.c_uint64),
+        ("cell_updates", ctypes.c_uint64),
     ]
 
 
@@ -107,6 +130,17 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.gol_fill_bernoulli.argtypes = [vp, u64, u64, ctypes.c_uint32, u64, ctypes.c_double]
     lib.gol_fill_patches.argtypes = [vp, u64, u64, ctypes.c_uint32, u64, ctypes.c_uint32,
                                      ctypes.c_uint32, ctypes.c_double]
+    lib.gol_batch_create.argtypes = [ctypes.POINTER(GolBatchConfig), ctypes.POINTER(vp)]
+    lib.gol_batch_add.argtypes = [vp, u64, u64, ctypes.c_uint32, u64, vp, ctypes.POINTER(ctypes.c_uint32)]
+    lib.gol_batch_run.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    lib.gol_batch_reset.argtypes = [vp]
+    lib.gol_batch_fetch.argtypes = [vp, ctypes.c_uint32, vp, ctypes.c_size_t]
+    lib.gol_batch_get_info.argtypes = [vp, ctypes.POINTER(GolBatchInfo)]
+    lib.gol_batch_destroy.argtypes = [vp]
+    lib.gol_batch_destroy.restype = None
+    for name in ("gol_batch_create", "gol_batch_add", "gol_batch_run", "gol_batch_reset",
+                 "gol_batch_fetch", "gol_batch_get_info"):
+        getattr(lib, name).restype = ctypes.c_int
     for name in ("gol_create", "gol_upload", "gol_upload_device", "gol_step", "gol_step_timed", "gol_download",
                  "gol_sync", "gol_device_state", "gol_get_stream", "gol_get_info",
                  "gol_fill_random", "gol_fill_bernoulli", "gol_fill_patches"):
@@ -178,6 +212,59 @@ class Grid:
     def close(self) -> None:
         if getattr(self, "handle", None):
             self.lib.gol_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many grids advanced together (gol_batch_*)."""
+
+    def __init__(self, device: int = -1, round_generations: int = 0, use_graph: bool = True):
+        self.lib = load_library()
+        cfg = GolBatchConfig(device, round_generations, 1 if use_graph else 0, 0)
+        h = ctypes.c_void_p()
+        _check(self.lib.gol_batch_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+        self.shapes = []
+
+    def add(self, words: np.ndarray, x_cells: int, y_cells: int, generations: int, word_bits: int = 64) -> int:
+        words = np.ascontiguousarray(words, dtype=np.uint64 if word_bits == 64 else np.uint32).reshape(-1)
+        gid = ctypes.c_uint32()
+        _check(self.lib.gol_batch_add(self.handle, x_cells, y_cells, word_bits, generations,
+                                      words.ctypes.data, ctypes.byref(gid)))
+        self.shapes.append((x_cells, y_cells, word_bits))
+        return gid.value
+
+    def run(self) -> float:
+        ms = ctypes.c_double()
+        _check(self.lib.gol_batch_run(self.handle, ctypes.byref(ms)))
+        return ms.value
+
+    def fetch(self, gid: int) -> np.ndarray:
+        x, y, wb = self.shapes[gid]
+        out = np.empty((x // wb) * y, dtype=np.uint64 if wb == 64 else np.uint32)
+        _check(self.lib.gol_batch_fetch(self.handle, gid, out.ctypes.data, out.size))
+        return out
+
+    def info(self) -> GolBatchInfo:
+        info = GolBatchInfo()
+        _check(self.lib.gol_batch_get_info(self.handle, ctypes.byref(info)))
+        return info
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.gol_batch_destroy(self.handle)
             self.handle = None
 
     def __enter__(self):

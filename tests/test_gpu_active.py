@@ -1,0 +1,105 @@
+"""GPU parity of change tracking (GOL_KERNEL_ACTIVE): bit-exact against the
+oracle and against the dense kernel, on mostly-stable, fully active and
+oscillating inputs, with generation counts that end on partial blocks."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+import golhip
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+FIX = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "fixtures.json")))
+
+
+def note(*a):
+    print("[case]", *a, flush=True)
+
+
+def run(words, x, y, gens, **kw):
+    with golhip.Grid(x, y, kernel=golhip.KERNEL_ACTIVE, **kw) as g:
+        g.upload(words)
+        g.step(gens)
+        return g.download(), g.info()
+
+
+@pytest.mark.parametrize("steps", [4, 12, 16])
+def test_patches_against_naive(steps):
+    x, y = 4096, 1024
+    w = golhip.fill_patches(x, y, 3, 12, 64, 0.35)
+    cells = ol.unpack(w, x, y)
+    for gens in (steps * 10, steps * 10 + 5, 301):
+        note("active patches", x, y, "steps", steps, "gens", gens)
+        out, info = run(w, x, y, gens, temporal_steps=steps)
+        ref = ol.naive_run(cells, gens)
+        assert np.array_equal(ol.unpack(out, x, y), ref)
+        assert info.tracked_blocks == gens // steps
+
+
+def test_skips_tiles_on_a_stable_grid():
+    x = y = 8192
+    w = golhip.fill_patches(x, y, 5, 4, 64, 0.35)
+    note("active skip", x, y)
+    out, info = run(w, x, y, 1200, temporal_steps=12)
+    tiles = info.tiles
+    blocks = info.tracked_blocks
+    assert blocks == 100
+    # after the first block almost all of the grid is quiet
+    assert info.active_tiles_total < 0.25 * tiles * blocks
+    ref = ol.bitpar_run(w, x, y, 1200)
+    assert np.array_equal(out, ref)
+
+
+def test_fully_active_random():
+    x, y = 2048, 512
+    c = ol.random_cells(x, y, 9)
+    note("active random", x, y)
+    out, _ = run(ol.pack(c), x, y, 123)
+    assert np.array_equal(ol.unpack(out, x, y), ol.naive_run(c, 123))
+
+
+def test_oscillators_stay_correct_between_blocks():
+    # blinkers every 4 cells (the reference's always-changing loader,
+    # data_loader.hpp:117-134): period 2, unchanged at every even block boundary
+    x, y = 4096, 512
+    c = np.zeros((y, x), np.uint8)
+    for yy in range(1, y - 2, 4):
+        for xx in range(0, x - 3, 4):
+            c[yy, xx:xx + 3] = 1
+    w = ol.pack(c)
+    for gens in (12 * 7, 12 * 7 + 1, 12 * 7 + 3):
+        note("active blinkers", gens)
+        out, info = run(w, x, y, gens, temporal_steps=12)
+        assert np.array_equal(ol.unpack(out, x, y), ol.naive_run(c, gens))
+    # odd block length: the blinkers change at every block boundary
+    out, info = run(w, x, y, 9 * 11, temporal_steps=9)
+    assert np.array_equal(ol.unpack(out, x, y), ol.naive_run(c, 99))
+    assert info.active_tiles_total == info.tiles * info.tracked_blocks
+
+
+def test_split_calls_and_reupload():
+    x, y = 2048, 600
+    w = golhip.fill_patches(x, y, 11, 6, 64, 0.35)
+    ref = ol.bitpar_run(w, x, y, 250)
+    with golhip.Grid(x, y, kernel=golhip.KERNEL_ACTIVE, temporal_steps=12) as g:
+        g.upload(w)
+        for k in (7, 24, 100, 1, 118):
+            g.step(k)
+        assert np.array_equal(g.download(), ref)
+        g.upload(w)
+        g.step(250)
+        assert np.array_equal(g.download(), ref)
+
+
+def test_full_size_patches_20000():
+    f = [r for r in FIX["bitpar_runs"] if r["name"] == "patches-16384" and r["generations"] == 20000][0]
+    p = f["params"]
+    w = golhip.fill_patches(16384, 16384, 42, p["n_patches"], p["patch"], p["p"])
+    note("active 16384^2 patches 20000")
+    out, info = run(w, 16384, 16384, 20000)
+    assert ol.sha256(out) == f["sha256"]
+    print("active fraction", info.active_tiles_total / (info.tiles * info.tracked_blocks))

@@ -1,0 +1,83 @@
+"""GPU parity of the batched engine: every grid of a mixed batch is bit-exact
+against the oracle and against the same grid run alone."""
+
+import numpy as np
+import pytest
+
+import golhip
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+
+def note(*a):
+    print("[case]", *a, flush=True)
+
+
+def mixed_specs(n, seed, max_side=1024, max_gens=300):
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(n):
+        side_x = int(64 * max(1, round(np.exp(rng.uniform(np.log(64), np.log(max_side))) / 64)))
+        side_y = int(64 * max(1, round(np.exp(rng.uniform(np.log(64), np.log(max_side))) / 64)))
+        if i % 5 == 0:
+            side_y = int(rng.integers(1, 200))  # ragged heights
+        gens = int(rng.integers(0, max_gens))
+        p = float(rng.uniform(0.2, 0.5))
+        specs.append((side_x, side_y, gens, p, 1000 + i))
+    return specs
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_mixed_batch_against_oracle(use_graph):
+    specs = mixed_specs(40, 1)
+    with golhip.Batch(use_graph=use_graph) as b:
+        ids = []
+        for x, y, gens, p, seed in specs:
+            ids.append(b.add(golhip.fill_bernoulli(x, y, seed, p), x, y, gens))
+        note("batch of", len(specs), "graph", use_graph)
+        b.run()
+        info = b.info()
+        assert info.grids == len(specs)
+        for gid, (x, y, gens, p, seed) in zip(ids, specs):
+            ref = ol.bitpar_run(golhip.fill_bernoulli(x, y, seed, p), x, y, gens)
+            assert np.array_equal(b.fetch(gid), ref), (x, y, gens)
+        # a second run restarts from the initial states
+        b.run()
+        x, y, gens, p, seed = specs[3]
+        assert np.array_equal(b.fetch(ids[3]), ol.bitpar_run(golhip.fill_bernoulli(x, y, seed, p), x, y, gens))
+
+
+def test_large_split_grids_in_batch():
+    specs = [(2048, 2048, 200, 0.5, 1), (1088, 1600, 97, 0.3, 2), (64, 2048, 150, 0.4, 3), (2048, 64, 64, 0.5, 4),
+             (1984, 1216, 33, 0.45, 5)]
+    with golhip.Batch(round_generations=16) as b:
+        for x, y, gens, p, seed in specs:
+            b.add(golhip.fill_bernoulli(x, y, seed, p), x, y, gens)
+        note("split grids")
+        b.run()
+        for i, (x, y, gens, p, seed) in enumerate(specs):
+            ref = ol.bitpar_run(golhip.fill_bernoulli(x, y, seed, p), x, y, gens)
+            assert np.array_equal(b.fetch(i), ref), (x, y, gens)
+
+
+def test_same_alone_and_in_the_mix():
+    specs = mixed_specs(12, 7, max_side=2048, max_gens=400)
+    with golhip.Batch() as b:
+        for x, y, gens, p, seed in specs:
+            b.add(golhip.fill_bernoulli(x, y, seed, p), x, y, gens)
+        b.run()
+        for i, (x, y, gens, p, seed) in enumerate(specs):
+            note("alone vs mix", i, x, y, gens)
+            with golhip.Grid(x, y) as g:
+                g.upload(golhip.fill_bernoulli(x, y, seed, p))
+                g.step(gens)
+                assert np.array_equal(g.download(), b.fetch(i))
+
+
+def test_word_bits_32_in_batch():
+    with golhip.Batch() as b:
+        c = ol.bernoulli_cells(96, 50, 3, 0.5)
+        gid = b.add(ol.pack(c, 32), 96, 50, 77, word_bits=32)
+        b.run()
+        assert np.array_equal(ol.unpack(b.fetch(gid), 96, 50, 32), ol.naive_run(c, 77))
