@@ -25,6 +25,7 @@ sys.path.insert(0, PKG)
 import numpy as np  # noqa: E402
 
 import golhip  # noqa: E402
+import workloads  # noqa: E402
 
 METRIC = "cell-updates/s at 16384² random; batched vs plain speedup; p50/p99 job latency"
 X = Y = 16384
@@ -59,6 +60,111 @@ def cpu_baseline(sample_generations: int) -> dict:
     }
 
 
+def fixture(name, gens):
+    fx = json.load(open(os.path.join(ROOT, "tests", "golden", "fixtures.json")))
+    for r in fx["bitpar_runs"]:
+        if r["name"] == name and r["generations"] == gens:
+            return r
+    return None
+
+
+def leg_mostly_stable() -> dict:
+    """16384^2 empty grid with 256 64x64 patches at density 0.35, 20000 generations:
+    change tracking against the dense temporal kernel on the same input."""
+    gens = 20000
+    w = golhip.fill_patches(X, Y, SEED, 256, 64, 0.35)
+    with golhip.Grid(X, Y, kernel=golhip.KERNEL_TEMPORAL, device=0) as g:
+        g.upload(w)
+        g.step_timed(16)  # warm-up launch
+        g.upload(w)
+        dense_ms = g.step_timed(gens)
+        dense_out = g.download()
+    with golhip.Grid(X, Y, kernel=golhip.KERNEL_ACTIVE, device=0) as g:
+        g.upload(w)
+        g.step_timed(24)
+        g.upload(w)
+        active_ms = g.step_timed(gens)
+        out = g.download()
+        info = g.info()
+    f = fixture("patches-16384", gens)
+    exact = hashlib.sha256(out.tobytes()).hexdigest() == f["sha256"] if f else None
+    if exact is False or not np.array_equal(out, dense_out):
+        raise SystemExit("mostly-stable: change-tracking result differs from the oracle / dense kernel")
+    blocks = max(1, info.tracked_blocks - 2)
+    return {
+        "workload": "16384x16384 empty, 256 patches 64x64 at p=0.35 (seed 42), 20000 generations",
+        "dense_equivalent_cell_updates_per_s": X * Y * gens / (active_ms / 1e3),
+        "active_ms": active_ms,
+        "dense_ms": dense_ms,
+        "speedup_vs_dense": dense_ms / active_ms,
+        "active_tile_fraction": info.active_tiles_total / (info.tiles * info.tracked_blocks),
+        "tiles": info.tiles,
+        "generations_per_block": info.temporal_steps,
+        "bit_exact_vs_oracle": exact,
+        "blocks_counted": blocks,
+    }
+
+
+def leg_batched(n_grids: int, plain_sample: int) -> dict:
+    """4096 uneven grids in one batch vs the same grids as plain per-grid launches
+    (one launch per generation; a seeded sample of grids is run that way)."""
+    specs = workloads.batched_uneven(n_grids)
+    t0 = time.perf_counter()
+    inputs = workloads.make_inputs(specs)
+    gen_s = time.perf_counter() - t0
+    total = sum(s.cell_updates for s in specs)
+    with golhip.Batch(device=0) as b:
+        for s, w in zip(specs, inputs):
+            b.add(w, s.side, s.side, s.generations)
+        first_ms = b.run()  # builds, uploads and records the graph
+        runs = [b.run() for _ in range(2)]
+        info = b.info()
+        batch_ms = min(runs)
+        rng = np.random.Generator(np.random.PCG64(99))
+        sample = sorted(rng.choice(len(specs), size=min(plain_sample, len(specs)), replace=False).tolist())
+        plain_wall = plain_gpu = alone_wall = 0.0
+        sample_updates = 0
+        for i in sample:
+            s = specs[i]
+            with golhip.Grid(s.side, s.side, kernel=golhip.KERNEL_PLAIN, device=0) as g:
+                g.upload(inputs[i])
+                g.sync()
+                t = time.perf_counter()
+                plain_gpu += g.step_timed(s.generations)
+                plain_wall += time.perf_counter() - t
+                res = g.download()
+            with golhip.Grid(s.side, s.side, device=0) as g:
+                g.upload(inputs[i])
+                g.sync()
+                t = time.perf_counter()
+                g.step_timed(s.generations)
+                alone_wall += time.perf_counter() - t
+            if not np.array_equal(res, b.fetch(i)):
+                raise SystemExit(f"batched grid {i} differs from the same grid run alone")
+            sample_updates += s.cell_updates
+    batched_rate = total / (batch_ms / 1e3)
+    plain_rate = sample_updates / plain_wall
+    alone_rate = sample_updates / alone_wall
+    return {
+        "workload": f"{n_grids} square grids, side log-uniform [64, 2048] (x64), p U[0.2,0.5], "
+                    f"generations U[100, 5000]",
+        "cell_updates": total,
+        "batched_ms": batch_ms,
+        "batched_cell_updates_per_s": batched_rate,
+        "first_run_ms_incl_build": first_ms,
+        "rounds": info.rounds,
+        "launches_in_graph": info.launches,
+        "plain_sample_grids": len(sample),
+        "plain_per_grid_cell_updates_per_s": plain_rate,
+        "plain_per_grid_gpu_cell_updates_per_s": sample_updates / (plain_gpu / 1e3),
+        "per_grid_temporal_cell_updates_per_s": alone_rate,
+        "speedup_vs_plain_per_grid": batched_rate / plain_rate,
+        "speedup_vs_per_grid_temporal": batched_rate / alone_rate,
+        "sample_bit_exact_alone_vs_batched": True,
+        "input_generation_s": gen_s,
+    }
+
+
 def expected_hash():
     fx = json.load(open(os.path.join(ROOT, "tests", "golden", "fixtures.json")))
     for r in fx["bitpar_runs"]:
@@ -73,9 +179,16 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--temporal-steps", type=int, default=0)
+    ap.add_argument("--lane-dwords", type=int, default=0)
+    ap.add_argument("--band-rows", type=int, default=0)
+    ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--kernel", default="temporal", choices=["temporal", "plain"])
     ap.add_argument("--cpu-sample-generations", type=int, default=150)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--legs", default="mostly_stable,batched",
+                    help="extra workloads reported beside the headline: mostly_stable,batched,served")
+    ap.add_argument("--batch-grids", type=int, default=4096)
+    ap.add_argument("--plain-sample", type=int, default=48)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the engine has no "
@@ -91,7 +204,8 @@ def main() -> int:
     src = golhip.Grid(X, Y, word_bits=64, kernel=golhip.KERNEL_PLAIN, device=0)
     src.upload(words)
     src.sync()
-    grid = golhip.Grid(X, Y, word_bits=64, kernel=kernel, temporal_steps=args.temporal_steps, device=0)
+    grid = golhip.Grid(X, Y, word_bits=64, kernel=kernel, temporal_steps=args.temporal_steps,
+                       lane_dwords=args.lane_dwords, band_rows=args.band_rows, waves=args.waves, device=0)
     info = grid.info()
     log(f"kernel {info.kernel} lane_dwords {info.lane_dwords} band_rows {info.band_rows} waves "
         f"{info.waves} temporal_steps {info.temporal_steps} tiles {info.tiles}")
@@ -166,11 +280,19 @@ def main() -> int:
         "gpu_ms_per_step": gpu_ms / args.steps,
         "avg_launch_us": avg_launch_s * 1e6,
     }
+    grid.close()
+    src.close()
+    legs = [l for l in args.legs.split(",") if l]
+    if "mostly_stable" in legs:
+        log("mostly-stable leg ...")
+        result["mostly_stable"] = leg_mostly_stable()
+    if "batched" in legs:
+        log("batched leg ...")
+        result["batched"] = leg_batched(args.batch_grids, args.plain_sample)
+        result["batched_vs_plain_speedup"] = result["batched"]["speedup_vs_plain_per_grid"]
     if not args.no_cpu_baseline:
         log("timing the CPU baseline sample ...")
         result["cpu_baseline"] = cpu_baseline(args.cpu_sample_generations)
-    grid.close()
-    src.close()
     print(json.dumps(result), flush=True)
     return 0
 

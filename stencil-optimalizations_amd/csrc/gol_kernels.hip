@@ -149,24 +149,72 @@ __global__ __launch_bounds__(64 * PLAIN_WAVES) void plain_kernel(const uint32_t*
 // launch, the active-tile launch and the batched work-item launch.
 // ---------------------------------------------------------------------------
 
+// LDS exchange of band edge rows, double-buffered by generation parity:
+//   tops[p][w]        top row of wave w      (tops[p][NW] stays zero)
+//   bottoms[p][w + 1] bottom row of wave w   (bottoms[p][0] stays zero)
+// so wave w reads bottoms[p][w] (row above) and tops[p][w + 1] (row below)
+// without a branch; the zero slots are the tile's top and bottom edges.
+template <int DW, int NW>
+struct EdgeRows {
+    uint32_t tops[2][NW + 1][64 * DW];
+    uint32_t bottoms[2][NW + 1][64 * DW];
+};
+
+template <int DW>
+__device__ __forceinline__ void lds_store_seg(uint32_t* row, int lane, const uint32_t (&v)[DW]) {
+    if constexpr (DW >= 4) {
+#pragma unroll
+        for (int k = 0; k < DW / 4; ++k)
+            reinterpret_cast<uint4*>(row + lane * DW)[k] = make_uint4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+    } else {
+#pragma unroll
+        for (int d = 0; d < DW; ++d) row[lane * DW + d] = v[d];
+    }
+}
+
+template <int DW>
+__device__ __forceinline__ void lds_load_seg(const uint32_t* row, int lane, uint32_t (&v)[DW]) {
+    if constexpr (DW >= 4) {
+#pragma unroll
+        for (int k = 0; k < DW / 4; ++k) {
+            const uint4 q = reinterpret_cast<const uint4*>(row + lane * DW)[k];
+            v[4 * k] = q.x;
+            v[4 * k + 1] = q.y;
+            v[4 * k + 2] = q.z;
+            v[4 * k + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < DW; ++d) v[d] = row[lane * DW + d];
+    }
+}
+
+template <int DW, int NW>
+__device__ __forceinline__ void clear_edge_slots(EdgeRows<DW, NW>& lds, int lane, int w) {
+    uint32_t z[DW];
+#pragma unroll
+    for (int d = 0; d < DW; ++d) z[d] = 0;
+    if (w == 0) {
+        lds_store_seg<DW>(lds.bottoms[0][0], lane, z);
+        lds_store_seg<DW>(lds.bottoms[1][0], lane, z);
+    }
+    if (w == NW - 1) {
+        lds_store_seg<DW>(lds.tops[0][NW], lane, z);
+        lds_store_seg<DW>(lds.tops[1][NW], lane, z);
+    }
+}
+
 template <int DW, int BR, int NW, bool MASK_ROWS, bool MASK_COLS>
-__device__ __forceinline__ void run_generations(uint32_t (&s)[BR][DW], uint32_t (*slots)[NW][2][64 * DW], int gens,
-                                                int lane, int w, int row_lo, int row_hi,
-                                                const uint32_t (&colmask)[DW]) {
+__device__ __forceinline__ void run_generations(uint32_t (&s)[BR][DW], EdgeRows<DW, NW>& lds, int gens, int lane,
+                                                int w, int row_lo, int row_hi, const uint32_t (&colmask)[DW]) {
     for (int g = 0; g < gens; ++g) {
         const int par = g & 1;
-#pragma unroll
-        for (int d = 0; d < DW; ++d) {
-            slots[par][w][0][lane * DW + d] = s[0][d];
-            slots[par][w][1][lane * DW + d] = s[BR - 1][d];
-        }
+        lds_store_seg<DW>(lds.tops[par][w], lane, s[0]);
+        lds_store_seg<DW>(lds.bottoms[par][w + 1], lane, s[BR - 1]);
         __syncthreads();
         uint32_t above[DW], below[DW];
-#pragma unroll
-        for (int d = 0; d < DW; ++d) {
-            above[d] = w > 0 ? slots[par][w - 1][1][lane * DW + d] : 0u;
-            below[d] = w < NW - 1 ? slots[par][w + 1][0][lane * DW + d] : 0u;
-        }
+        lds_load_seg<DW>(lds.bottoms[par][w], lane, above);
+        lds_load_seg<DW>(lds.tops[par][w + 1], lane, below);
         advance_band<DW, BR, MASK_ROWS, MASK_COLS>(s, above, below, row_lo, row_hi, colmask);
     }
 }
@@ -175,8 +223,7 @@ __device__ __forceinline__ void run_generations(uint32_t (&s)[BR][DW], uint32_t 
 // when TRACK is set.
 template <int DW, int BR, int NW, bool TRACK>
 __device__ __forceinline__ bool process_tile(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
-                                             const TileGeo& a, int tile, int gens,
-                                             uint32_t (*slots)[NW][2][64 * DW]) {
+                                             const TileGeo& a, int tile, int gens, EdgeRows<DW, NW>& lds) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const int tx = tile % a.tiles_x;
@@ -190,6 +237,7 @@ __device__ __forceinline__ bool process_tile(const uint32_t* __restrict__ src, u
     uint32_t s[BR][DW];
 #pragma unroll
     for (int r = 0; r < BR; ++r) load_seg<DW>(src, a.x_dw, a.y_rows, gy0 + r, gx0, vec_ok, s[r]);
+    clear_edge_slots<DW, NW>(lds, lane, w);
 
     uint32_t colmask[DW];
 #pragma unroll
@@ -201,11 +249,11 @@ __device__ __forceinline__ bool process_tile(const uint32_t* __restrict__ src, u
     const bool edge_y = y0 < 0 || y0 + NW * BR > a.y_rows;  // workgroup-uniform
     const bool edge_x = x0 < 0 || x0 + 64 * DW > a.x_dw;
     if (edge_x) {
-        run_generations<DW, BR, NW, true, true>(s, slots, gens, lane, w, row_lo, row_hi, colmask);
+        run_generations<DW, BR, NW, true, true>(s, lds, gens, lane, w, row_lo, row_hi, colmask);
     } else if (edge_y) {
-        run_generations<DW, BR, NW, true, false>(s, slots, gens, lane, w, row_lo, row_hi, colmask);
+        run_generations<DW, BR, NW, true, false>(s, lds, gens, lane, w, row_lo, row_hi, colmask);
     } else {
-        run_generations<DW, BR, NW, false, false>(s, slots, gens, lane, w, row_lo, row_hi, colmask);
+        run_generations<DW, BR, NW, false, false>(s, lds, gens, lane, w, row_lo, row_hi, colmask);
     }
 
     const int tile_rows = NW * BR;
@@ -231,8 +279,8 @@ __device__ __forceinline__ bool process_tile(const uint32_t* __restrict__ src, u
 template <int DW, int BR, int NW>
 __global__ __launch_bounds__(64 * NW) void tile_kernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
                                                        TileGeo a, int gens) {
-    __shared__ uint32_t slots[2][NW][2][64 * DW];
-    process_tile<DW, BR, NW, false>(src, dst, a, blockIdx.x, gens, slots);
+    __shared__ EdgeRows<DW, NW> lds;
+    process_tile<DW, BR, NW, false>(src, dst, a, blockIdx.x, gens, lds);
 }
 
 // Change tracking: workgroup i advances tile list[i] (i < *count) and raises
@@ -243,10 +291,10 @@ __global__ __launch_bounds__(64 * NW) void active_tile_kernel(const uint32_t* __
                                                               const int* __restrict__ list,
                                                               const int* __restrict__ count,
                                                               uint32_t* __restrict__ changed) {
-    __shared__ uint32_t slots[2][NW][2][64 * DW];
+    __shared__ EdgeRows<DW, NW> lds;
     if (static_cast<int>(blockIdx.x) >= *count) return;  // whole workgroup leaves before any barrier
     const int tile = list[blockIdx.x];
-    const bool c = process_tile<DW, BR, NW, true>(src, dst, a, tile, gens, slots);
+    const bool c = process_tile<DW, BR, NW, true>(src, dst, a, tile, gens, lds);
     if (__ballot(c) != 0 && (threadIdx.x & 63) == 0) changed[tile] = 1u;
 }
 
@@ -254,12 +302,12 @@ __global__ __launch_bounds__(64 * NW) void active_tile_kernel(const uint32_t* __
 template <int DW, int BR, int NW>
 __global__ __launch_bounds__(64 * NW) void item_kernel(const BatchGrid* __restrict__ grids,
                                                        const WorkItem* __restrict__ items) {
-    __shared__ uint32_t slots[2][NW][2][64 * DW];
+    __shared__ EdgeRows<DW, NW> lds;
     const WorkItem it = items[blockIdx.x];
     const BatchGrid& g = grids[it.grid];
     const uint32_t* src = g.buf[it.parity];
     uint32_t* dst = g.buf[it.parity ^ 1];
-    process_tile<DW, BR, NW, false>(src, dst, g.geo, it.tile, it.gens, slots);
+    process_tile<DW, BR, NW, false>(src, dst, g.geo, it.tile, it.gens, lds);
 }
 
 // Active-tile list for the next block: tile t is active when it or one of its
@@ -290,7 +338,8 @@ __global__ void build_active_list(const uint32_t* __restrict__ changed_prev, uin
 }  // namespace
 
 const TileShape kTileShapes[] = {
-    {8, 8, 8}, {8, 4, 16}, {4, 8, 8}, {4, 8, 16}, {2, 16, 8}, {1, 8, 8}, {1, 16, 8}, {1, 32, 16},
+    {8, 8, 8}, {8, 10, 8}, {8, 5, 16}, {8, 4, 16}, {4, 8, 8}, {4, 8, 16}, {2, 16, 8}, {1, 8, 8}, {1, 16, 8},
+    {1, 32, 16},
 };
 
 TileGeo to_device_geo(const TileGeometry& g, int64_t x_dw, int64_t y_rows) {
@@ -345,6 +394,8 @@ bool plan_tiles(const TileShape& shape, int64_t x_dw, int64_t y_rows, int max_ge
 
 #define GOLHIP_FOR_EACH_SHAPE(X) \
     X(8, 8, 8)                     \
+    X(8, 10, 8)                    \
+    X(8, 5, 16)                    \
     X(8, 4, 16)                    \
     X(4, 8, 8)                     \
     X(4, 8, 16)                    \
