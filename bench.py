@@ -165,6 +165,83 @@ def leg_batched(n_grids: int, plain_sample: int) -> dict:
     }
 
 
+def leg_served(n_jobs: int, load: float, throughput_hint: float) -> dict:
+    """Poisson arrivals of jobs (side log-uniform [128, 8192], p U[0.1, 0.5],
+    generations log-uniform [100, 50000]) into the continuous-batching server at
+    a rate that loads the GPU to `load` of `throughput_hint` cell-updates/s."""
+    import threading
+
+    probe = workloads.served_stream(n_jobs, 1.0)
+    mean_work = sum(s.cell_updates for s in probe) / len(probe)
+    rate = load * throughput_hint / mean_work
+    specs = workloads.served_stream(n_jobs, rate)
+    inputs = workloads.make_inputs(specs)
+    lat = [0.0] * n_jobs
+    queue_ms = [0.0] * n_jobs
+    ids = [None] * n_jobs
+    submitted = threading.Event()
+    with golhip.Server(device=0, hbm_budget_bytes=32 << 30) as srv:
+        # warm-up: one small job through the whole path
+        srv.wait(srv.submit(golhip.fill_random(256, 256, 1), 256, 256, 10))
+        t0 = time.perf_counter()
+
+        failure = []
+
+        def submitter():
+            try:
+                for i, s in enumerate(specs):
+                    delay = t0 + s.arrival_s - time.perf_counter()
+                    if delay > 0:
+                        time.sleep(delay)
+                    ids[i] = srv.submit(inputs[i], s.side, s.side, s.generations)
+            except Exception as e:  # surfaced by the waiting loop below
+                failure.append(e)
+            submitted.set()
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        results = {}
+        for i in range(n_jobs):
+            while ids[i] is None:
+                if failure:
+                    raise SystemExit(f"served leg: submitting job {i} failed: {failure[0]}")
+                time.sleep(0.0005)
+            out, st = srv.wait(ids[i])
+            lat[i] = st.latency_ms
+            queue_ms[i] = st.queue_ms
+            results[i] = out
+        th.join()
+        wall = time.perf_counter() - t0
+        stats = srv.stats()
+    total = sum(s.cell_updates for s in specs)
+    # same result alone as in the mix, on a seeded sample of jobs
+    rng = np.random.Generator(np.random.PCG64(5))
+    sample = sorted(rng.choice(n_jobs, size=min(12, n_jobs), replace=False).tolist())
+    for i in sample:
+        s = specs[i]
+        with golhip.Grid(s.side, s.side, device=0) as g:
+            g.upload(inputs[i])
+            g.step(s.generations)
+            if not np.array_equal(g.download(), results[i]):
+                raise SystemExit(f"served job {i} differs from the same job run alone")
+    lat_sorted = sorted(lat)
+    return {
+        "workload": f"{n_jobs} jobs, Poisson arrivals at {rate:.1f}/s (target load {load:.2f} of "
+                    f"{throughput_hint:.2e} cell-updates/s), side log-uniform [128, 8192] (x64), "
+                    f"p U[0.1,0.5], generations log-uniform [100, 50000]",
+        "cell_updates": total,
+        "wall_s": wall,
+        "cell_updates_per_s": total / wall,
+        "latency_p50_ms": float(np.percentile(lat_sorted, 50)),
+        "latency_p99_ms": float(np.percentile(lat_sorted, 99)),
+        "latency_max_ms": lat_sorted[-1],
+        "queue_p50_ms": float(np.percentile(queue_ms, 50)),
+        "rounds": stats.rounds,
+        "resident_bytes_peak": stats.resident_bytes_peak,
+        "sample_bit_exact_alone_vs_served": len(sample),
+    }
+
+
 def expected_hash():
     fx = json.load(open(os.path.join(ROOT, "tests", "golden", "fixtures.json")))
     for r in fx["bitpar_runs"]:
@@ -183,12 +260,14 @@ def main() -> int:
     ap.add_argument("--band-rows", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
     ap.add_argument("--kernel", default="temporal", choices=["temporal", "plain"])
-    ap.add_argument("--cpu-sample-generations", type=int, default=150)
+    ap.add_argument("--cpu-sample-generations", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--legs", default="mostly_stable,batched",
+    ap.add_argument("--legs", default="mostly_stable,batched,served",
                     help="extra workloads reported beside the headline: mostly_stable,batched,served")
     ap.add_argument("--batch-grids", type=int, default=4096)
     ap.add_argument("--plain-sample", type=int, default=48)
+    ap.add_argument("--served-jobs", type=int, default=400)
+    ap.add_argument("--served-load", type=float, default=0.8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the engine has no "
@@ -290,6 +369,12 @@ def main() -> int:
         log("batched leg ...")
         result["batched"] = leg_batched(args.batch_grids, args.plain_sample)
         result["batched_vs_plain_speedup"] = result["batched"]["speedup_vs_plain_per_grid"]
+    if "served" in legs:
+        log("served leg ...")
+        hint = result.get("batched", {}).get("batched_cell_updates_per_s", 2.5e13)
+        result["served"] = leg_served(args.served_jobs, args.served_load, hint)
+        result["latency_p50_ms"] = result["served"]["latency_p50_ms"]
+        result["latency_p99_ms"] = result["served"]["latency_p99_ms"]
     if not args.no_cpu_baseline:
         log("timing the CPU baseline sample ...")
         result["cpu_baseline"] = cpu_baseline(args.cpu_sample_generations)

@@ -169,6 +169,62 @@ int gol_batch_fetch(gol_batch* b, uint32_t grid_id, void* words, size_t n_words)
 int gol_batch_get_info(gol_batch* b, gol_batch_info* info);
 void gol_batch_destroy(gol_batch* b);
 
+/* ---- Server: continuous batching of simulation jobs (no reference
+ * counterpart). A scheduler thread inside the library advances all resident
+ * jobs in rounds; between rounds it admits queued jobs (arrival order) whose
+ * packed state fits the HBM budget, so requests that arrive while others run
+ * join the running batch. Results are bit-identical to running each job alone.
+ * gol_server_submit / _poll / _wait / _get_stats may be called from any
+ * thread; gol_server_destroy only after every wait has returned. ---- */
+typedef struct gol_server gol_server;
+
+typedef struct gol_server_config {
+    int32_t device;             /* HIP device ordinal; -1 = current device */
+    uint32_t round_generations; /* generations per round for grids split into tiles (1..32); 0 = 32 */
+    uint64_t hbm_budget_bytes;  /* device bytes for resident job state (2 buffers per job); 0 = 8 GiB */
+    uint32_t max_jobs;          /* resident jobs at most; 0 = 4096 */
+    uint32_t reserved;
+} gol_server_config;
+
+enum gol_job_state {
+    GOL_JOB_QUEUED = 0,   /* waiting for HBM budget */
+    GOL_JOB_RESIDENT = 1, /* state on the device, advancing every round */
+    GOL_JOB_DRAINING = 2, /* finished, result being copied out */
+    GOL_JOB_DONE = 3,
+    GOL_JOB_FAILED = 4    /* e.g. GOL_ERR_OUT_OF_MEMORY: larger than the whole budget */
+};
+
+typedef struct gol_job_stats {
+    double queue_ms;   /* submit -> admitted */
+    double latency_ms; /* submit -> result in host memory */
+    uint32_t rounds;   /* rounds the job took part in */
+    uint32_t reserved;
+} gol_job_stats;
+
+typedef struct gol_server_stats {
+    uint64_t submitted, admitted, completed, rejected;
+    uint64_t rounds, items;
+    uint64_t cell_updates;        /* cells x generations enqueued so far */
+    uint64_t resident_bytes;      /* arena bytes in use now */
+    uint64_t resident_bytes_peak;
+    uint32_t resident_jobs, queued_jobs;
+} gol_server_stats;
+
+/* config may be NULL (defaults above). Allocates the budget up front. */
+int gol_server_create(const gol_server_config* config, gol_server** out);
+/* Queue a job: copies the input words; returns at once with its id. */
+int gol_server_submit(gol_server* s, uint64_t x_cells, uint64_t y_cells, uint32_t word_bits, uint64_t generations,
+                      const void* words, uint64_t* job_id);
+/* Current enum gol_job_state of a job not yet waited for. */
+int gol_server_poll(gol_server* s, uint64_t job_id, int* state);
+/* Block until the job is done, copy its final state into words (may be NULL)
+ * and its timings into stats (may be NULL), and forget the job. Returns the
+ * job's error code if it failed. */
+int gol_server_wait(gol_server* s, uint64_t job_id, void* words, size_t n_words, gol_job_stats* stats);
+int gol_server_get_stats(gol_server* s, gol_server_stats* stats);
+/* Stop the scheduler (jobs not finished fail) and free everything. */
+void gol_server_destroy(gol_server* s);
+
 /* Host-side input generators (no device work), writing packed row words:
  *  gol_fill_random: the reference's RandomOnesZerosDataLoader
  *    (src/infrastructure/data_loader.hpp:57-75): std::mt19937(seed),

@@ -76,6 +76,56 @@ void set_tuned_shape(int64_t x_dw, int64_t y_rows, const TileShape& s) {
     g_tuned[size_class(x_dw, y_rows)] = s;
 }
 
+// Cost per generation of advancing a whole grid split into tiles, in CU
+// microseconds, when many grids share the GPU (no wave quantization): the
+// compute of every tile row (about 13.5 VALU ops per dword at 64 lanes per
+// clock, 2.4 GHz) plus, once per launch, loading each tile with its halo and
+// storing its interior (about 5 TB/s shared by 256 CUs).
+static double split_cost_per_generation(const TileShape& s, const TileGeometry& g, int steps) {
+    const double tiles = static_cast<double>(g.tiles_x) * g.tiles_y;
+    const double width = 64.0 * s.lane_dwords;
+    const double compute = tiles * g.tile_rows * width * 13.5 / 64.0 / 2400.0;
+    const double memory = tiles * (static_cast<double>(g.tile_rows) + g.stride_rows) * width * 4.0 * 256.0 / 5.0e6;
+    return compute + memory / steps;
+}
+
+int plan_batch_grid(int64_t x_dw, int64_t y_rows, int round_generations, TileShape* shape, TileGeometry* geom,
+                    int* per_round) {
+    constexpr int kBigTileRows = 512;
+    const char* why = nullptr;
+    // grids that fit one tile of their size class need no halo
+    const TileShape table = batch_shape(x_dw, y_rows);
+    if (x_dw <= 64 * table.lane_dwords && y_rows <= table.band_rows * table.waves) {
+        *shape = table;
+        *per_round = chunk_generations(table, round_generations, kBigTileRows);
+        if (!plan_tiles(table, x_dw, y_rows, 1, geom, &why))
+            return fail(GOL_ERR_RUNTIME, "grid %lldx%lld: %s", static_cast<long long>(x_dw * 32),
+                        static_cast<long long>(y_rows), why);
+        return GOL_OK;
+    }
+    // split grids: the compiled shape and halo depth (<= round_generations)
+    // with the lowest cost per generation
+    double best = 1e300;
+    for (int i = 0; i < kNumTileShapes; ++i) {
+        const TileShape& s = kTileShapes[i];
+        for (int steps = round_generations; steps >= 1; --steps) {
+            TileGeometry g;
+            if (!plan_tiles(s, x_dw, y_rows, steps, &g, &why)) continue;
+            const double c = split_cost_per_generation(s, g, steps);
+            if (c < best) {
+                best = c;
+                *shape = s;
+                *geom = g;
+                *per_round = steps;
+            }
+        }
+    }
+    if (best >= 1e300)
+        return fail(GOL_ERR_RUNTIME, "grid %lldx%lld: no compiled tile shape fits", static_cast<long long>(x_dw * 32),
+                    static_cast<long long>(y_rows));
+    return GOL_OK;
+}
+
 }  // namespace golhip
 
 struct BatchEntry {
@@ -132,17 +182,10 @@ struct Guard {
 // Plan shapes, rounds and items; allocate the arena and upload everything.
 int build(gol_batch* b) {
     const int T = b->cfg.round_generations ? static_cast<int>(b->cfg.round_generations) : 32;
-    const int big_rows = 512;
     size_t off = 0;
     for (auto& g : b->grids) {
-        g.shape = batch_shape(g.x_dw, g.y_rows);
-        const bool one_tile = g.x_dw <= 64 * g.shape.lane_dwords && g.y_rows <= g.shape.band_rows * g.shape.waves;
-        const char* why = nullptr;
-        int per_round = one_tile ? chunk_generations(g.shape, T, big_rows) : T;
-        if (!plan_tiles(g.shape, g.x_dw, g.y_rows, one_tile ? 1 : per_round, &g.geom, &why))
-            return fail(GOL_ERR_RUNTIME, "batch grid %lldx%lld: %s", static_cast<long long>(g.x_dw * 32),
-                        static_cast<long long>(g.y_rows), why);
-        g.chunk = per_round;
+        int rc = plan_batch_grid(g.x_dw, g.y_rows, T, &g.shape, &g.geom, &g.chunk);
+        if (rc) return rc;
         g.offset = off;
         const size_t n = static_cast<size_t>(g.x_dw * g.y_rows);
         off += 3 * ((n + 63) & ~size_t(63));  // buf0, buf1, initial copy; 256-byte aligned

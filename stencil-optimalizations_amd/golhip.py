@@ -38,7 +38,11 @@ EXPORTED_SYMBOLS = (
     "gol_version", "gol_fill_random", "gol_fill_bernoulli", "gol_fill_patches",
     "gol_batch_create", "gol_batch_add", "gol_batch_run", "gol_batch_reset", "gol_batch_fetch",
     "gol_batch_get_info", "gol_batch_destroy",
+    "gol_server_create", "gol_server_submit", "gol_server_poll", "gol_server_wait",
+    "gol_server_get_stats", "gol_server_destroy",
 )
+
+JOB_QUEUED, JOB_RESIDENT, JOB_DRAINING, JOB_DONE, JOB_FAILED = range(5)
 
 
 class GolError(RuntimeError):
@@ -98,6 +102,36 @@ class GolBatchInfo(ctypes.Structure):
     ]
 
 
+class GolServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int32),
+        ("round_generations", ctypes.c_uint32),
+        ("hbm_budget_bytes", ctypes.c_uint64),
+        ("max_jobs", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class GolJobStats(ctypes.Structure):
+    _fields_ = [
+        ("queue_ms", ctypes.c_double),
+        ("latency_ms", ctypes.c_double),
+        ("rounds", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class GolServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64), ("admitted", ctypes.c_uint64),
+        ("completed", ctypes.c_uint64), ("rejected", ctypes.c_uint64),
+        ("rounds", ctypes.c_uint64), ("items", ctypes.c_uint64),
+        ("cell_updates", ctypes.c_uint64),
+        ("resident_bytes", ctypes.c_uint64), ("resident_bytes_peak", ctypes.c_uint64),
+        ("resident_jobs", ctypes.c_uint32), ("queued_jobs", ctypes.c_uint32),
+    ]
+
+
 _lib: Optional[ctypes.CDLL] = None
 
 
@@ -138,6 +172,16 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.gol_batch_get_info.argtypes = [vp, ctypes.POINTER(GolBatchInfo)]
     lib.gol_batch_destroy.argtypes = [vp]
     lib.gol_batch_destroy.restype = None
+    lib.gol_server_create.argtypes = [ctypes.POINTER(GolServerConfig), ctypes.POINTER(vp)]
+    lib.gol_server_submit.argtypes = [vp, u64, u64, ctypes.c_uint32, u64, vp, ctypes.POINTER(u64)]
+    lib.gol_server_poll.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_int)]
+    lib.gol_server_wait.argtypes = [vp, u64, vp, ctypes.c_size_t, ctypes.POINTER(GolJobStats)]
+    lib.gol_server_get_stats.argtypes = [vp, ctypes.POINTER(GolServerStats)]
+    lib.gol_server_destroy.argtypes = [vp]
+    lib.gol_server_destroy.restype = None
+    for name in ("gol_server_create", "gol_server_submit", "gol_server_poll", "gol_server_wait",
+                 "gol_server_get_stats"):
+        getattr(lib, name).restype = ctypes.c_int
     for name in ("gol_batch_create", "gol_batch_add", "gol_batch_run", "gol_batch_reset",
                  "gol_batch_fetch", "gol_batch_get_info"):
         getattr(lib, name).restype = ctypes.c_int
@@ -265,6 +309,63 @@ class Batch:
     def close(self) -> None:
         if getattr(self, "handle", None):
             self.lib.gol_batch_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching job server (gol_server_*). submit/poll/wait are thread-safe."""
+
+    def __init__(self, device: int = -1, round_generations: int = 0, hbm_budget_bytes: int = 0,
+                 max_jobs: int = 0):
+        self.lib = load_library()
+        cfg = GolServerConfig(device, round_generations, hbm_budget_bytes, max_jobs, 0)
+        h = ctypes.c_void_p()
+        _check(self.lib.gol_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+        self._shapes = {}
+
+    def submit(self, words: np.ndarray, x_cells: int, y_cells: int, generations: int,
+               word_bits: int = 64) -> int:
+        words = np.ascontiguousarray(words, dtype=np.uint64 if word_bits == 64 else np.uint32).reshape(-1)
+        jid = ctypes.c_uint64()
+        _check(self.lib.gol_server_submit(self.handle, x_cells, y_cells, word_bits, generations,
+                                          words.ctypes.data, ctypes.byref(jid)))
+        self._shapes[jid.value] = (x_cells, y_cells, word_bits)
+        return jid.value
+
+    def poll(self, jid: int) -> int:
+        st = ctypes.c_int()
+        _check(self.lib.gol_server_poll(self.handle, jid, ctypes.byref(st)))
+        return st.value
+
+    def wait(self, jid: int):
+        """Returns (final words, GolJobStats)."""
+        x, y, wb = self._shapes.pop(jid)
+        out = np.empty((x // wb) * y, dtype=np.uint64 if wb == 64 else np.uint32)
+        st = GolJobStats()
+        _check(self.lib.gol_server_wait(self.handle, jid, out.ctypes.data, out.size, ctypes.byref(st)))
+        return out, st
+
+    def stats(self) -> GolServerStats:
+        st = GolServerStats()
+        _check(self.lib.gol_server_get_stats(self.handle, ctypes.byref(st)))
+        return st
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.gol_server_destroy(self.handle)
             self.handle = None
 
     def __enter__(self):

@@ -50,7 +50,7 @@ def test_mixed_batch_against_oracle(use_graph):
 
 def test_large_split_grids_in_batch():
     specs = [(2048, 2048, 200, 0.5, 1), (1088, 1600, 97, 0.3, 2), (64, 2048, 150, 0.4, 3), (2048, 64, 64, 0.5, 4),
-             (1984, 1216, 33, 0.45, 5)]
+             (1984, 1216, 33, 0.45, 5), (7104, 900, 41, 0.35, 6), (4160, 3000, 20, 0.4, 7)]
     with golhip.Batch(round_generations=16) as b:
         for x, y, gens, p, seed in specs:
             b.add(golhip.fill_bernoulli(x, y, seed, p), x, y, gens)
