@@ -1,0 +1,173 @@
+// MI355X algorithm plug-ins for the reference's Algorithm<2, cell_t> interface
+// (src/infrastructure/algorithm.hpp:17-87), driving the engine through the C ABI
+// of include/golhip.h. Compiles with any C++17 compiler: no HIP headers.
+//
+// Registered names (register_hip_algorithms):
+//   gol-hip-rows-32 / -64       one generation per launch
+//                               (the role of cuda-naive-bitwise, cuda_naive_bitwise_kernel.cu:32-79)
+//   gol-hip-temporal-32 / -64   temporal tiles, several generations per launch
+//                               (the role of cuda-temporal-rowed, temporal_rowed.cu:33-229)
+//   gol-hip-local-32 / -64      change tracking over active tiles
+//                               (the role of cuda-naive-local-one-cell, cuda_local_one_cell.cu:53-242)
+//   gol-hip-auto-64             shape and depth chosen by the library's time model
+//
+// Parameters read from ExperimentParams: temporal_steps (> 1: generations per
+// launch; 0 or 1: chosen by the library), thread_block_size (> 0: wavefronts
+// per workgroup = thread_block_size / 64), max_runtime_seconds.
+// Errors follow the reference: std::invalid_argument for a width that is not a
+// multiple of the word size (general_bit_grid.hpp:208-215), std::runtime_error
+// for unusable configurations, HipError (the CudaError of cuch.hpp:42-52) for
+// device failures.
+#pragma once
+
+#include <algorithm>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../../../include/golhip.h"
+#include "algorithm.hpp"
+#include "algorithm_repository.hpp"
+#include "grid.hpp"
+#include "timer.hpp"
+
+namespace algorithms {
+
+class HipError : public std::runtime_error {
+   public:
+    using std::runtime_error::runtime_error;
+};
+
+inline void check_gol(int rc) {
+    if (rc == GOL_OK) return;
+    const std::string msg = gol_last_error();
+    switch (rc) {
+        case GOL_ERR_INVALID_ARGUMENT:
+            throw std::invalid_argument(msg);
+        case GOL_ERR_RUNTIME:
+        case GOL_ERR_NOT_FOUND:
+        case GOL_ERR_BUSY:
+            throw std::runtime_error(msg);
+        default:
+            throw HipError(msg);
+    }
+}
+
+// Row packing (FullyPackedRows, bit_modes.hpp:127-163): word (wx, y) at
+// y * (X / Bits) + wx, bit k = cell (Bits * wx + k, y); cells > 0 are alive.
+template <std::size_t Bits>
+struct RowWords {
+    using word_type = std::conditional_t<Bits == 64, std::uint64_t, std::uint32_t>;
+
+    template <typename cell_t>
+    static std::vector<word_type> pack(const infrastructure::Grid<2, cell_t>& grid) {
+        const std::size_t X = grid.template size_in<0>(), Y = grid.template size_in<1>();
+        if (X % Bits != 0)
+            throw std::invalid_argument("Grid dimensions X must be a multiple of " + std::to_string(Bits));
+        std::vector<word_type> words(X / Bits * Y, 0);
+        const cell_t* d = grid.data();
+        for (std::size_t y = 0; y < Y; ++y)
+            for (std::size_t x = 0; x < X; ++x)
+                if (d[y * X + x] > 0) words[y * (X / Bits) + x / Bits] |= word_type{1} << (x % Bits);
+        return words;
+    }
+
+    template <typename cell_t>
+    static infrastructure::Grid<2, cell_t> unpack(const std::vector<word_type>& words, std::size_t X, std::size_t Y) {
+        infrastructure::Grid<2, cell_t> grid(X, Y);
+        cell_t* d = grid.data();
+        for (std::size_t y = 0; y < Y; ++y)
+            for (std::size_t x = 0; x < X; ++x)
+                d[y * X + x] = static_cast<cell_t>((words[y * (X / Bits) + x / Bits] >> (x % Bits)) & 1);
+        return grid;
+    }
+};
+
+template <typename grid_cell_t, std::size_t Bits, uint32_t Kernel>
+class GoLHip : public infrastructure::Algorithm<2, grid_cell_t> {
+   public:
+    using size_type = std::size_t;
+    using DataGrid = infrastructure::Grid<2, grid_cell_t>;
+    using Words = RowWords<Bits>;
+
+    GoLHip() = default;
+    ~GoLHip() override { release(); }
+
+    void set_and_format_input_data(const DataGrid& data) override {
+        x_ = data.template size_in<0>();
+        y_ = data.template size_in<1>();
+        words_ = Words::template pack<grid_cell_t>(data);
+    }
+
+    void initialize_data_structures() override {
+        release();
+        gol_config c{};
+        c.x_cells = x_;
+        c.y_cells = y_;
+        c.word_bits = static_cast<uint32_t>(Bits);
+        c.kernel = Kernel;
+        c.temporal_steps = this->params.temporal_steps > 1 ? static_cast<uint32_t>(this->params.temporal_steps) : 0;
+        c.waves = this->params.thread_block_size >= 64 ? static_cast<uint32_t>(this->params.thread_block_size / 64) : 0;
+        c.device = -1;
+        check_gol(gol_create(&c, &handle_));
+        check_gol(gol_upload(handle_, words_.data(), words_.size()));
+    }
+
+    void run(size_type iterations) override {
+        performed_ = 0;
+        if (this->params.max_runtime_seconds >= static_cast<std::size_t>(infrastructure::StopWatch::MAX_TIME)) {
+            check_gol(gol_step(handle_, iterations));
+            check_gol(gol_sync(handle_));
+            performed_ = iterations;
+            return;
+        }
+        // time budget: advance in chunks and stop once it is spent
+        infrastructure::StopWatch watch(this->params.max_runtime_seconds);
+        gol_info info{};
+        check_gol(gol_get_info(handle_, &info));
+        const size_type chunk = std::max<size_type>(1, static_cast<size_type>(info.temporal_steps) * 16);
+        while (performed_ < iterations && !watch.time_is_up()) {
+            const size_type n = std::min(chunk, iterations - performed_);
+            check_gol(gol_step(handle_, n));
+            check_gol(gol_sync(handle_));
+            performed_ += n;
+        }
+    }
+
+    void finalize_data_structures() override {
+        check_gol(gol_download(handle_, words_.data(), words_.size()));
+        release();
+    }
+
+    DataGrid fetch_result() override { return Words::template unpack<grid_cell_t>(words_, x_, y_); }
+
+    std::size_t actually_performed_iterations() const override { return performed_; }
+
+   private:
+    void release() {
+        if (handle_) {
+            gol_destroy(handle_);
+            handle_ = nullptr;
+        }
+    }
+
+    gol_handle* handle_ = nullptr;
+    std::vector<typename Words::word_type> words_;
+    size_type x_ = 0, y_ = 0;
+    size_type performed_ = 0;
+};
+
+template <typename cell_t>
+void register_hip_algorithms(infrastructure::AlgorithmRepository<2, cell_t>& repo) {
+    repo.template register_algorithm<GoLHip<cell_t, 32, GOL_KERNEL_PLAIN>>("gol-hip-rows-32");
+    repo.template register_algorithm<GoLHip<cell_t, 64, GOL_KERNEL_PLAIN>>("gol-hip-rows-64");
+    repo.template register_algorithm<GoLHip<cell_t, 32, GOL_KERNEL_TEMPORAL>>("gol-hip-temporal-32");
+    repo.template register_algorithm<GoLHip<cell_t, 64, GOL_KERNEL_TEMPORAL>>("gol-hip-temporal-64");
+    repo.template register_algorithm<GoLHip<cell_t, 32, GOL_KERNEL_ACTIVE>>("gol-hip-local-32");
+    repo.template register_algorithm<GoLHip<cell_t, 64, GOL_KERNEL_ACTIVE>>("gol-hip-local-64");
+    repo.template register_algorithm<GoLHip<cell_t, 64, GOL_KERNEL_AUTO>>("gol-hip-auto-64");
+}
+
+}  // namespace algorithms

@@ -1,0 +1,63 @@
+"""The C++ plug-in boundary (Algorithm<2, cell_t> mirror + HIP plug-ins) and the
+reference-compatible `stencils` driver, exercised through their compiled binaries."""
+
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_plugins")
+STENCILS = os.path.join(ROOT, "stencil-optimalizations_amd", "stencils")
+
+
+def run(args, timeout=240):
+    p = subprocess.run(args, capture_output=True, text=True, timeout=timeout)
+    return p.returncode, p.stdout + p.stderr
+
+
+def test_plugins_host_side():
+    rc, out = run([TEST_BIN, "cpu"])
+    assert rc == 0, out
+
+
+def test_cli_rejects_bad_flags():
+    rc, out = run([STENCILS, "--algorithm=gol-hip-rows-64", "--grid-dimensions-x=64"])
+    assert rc != 0 and "required" in out
+    rc, out = run([STENCILS, "--bogus=1"])
+    assert rc != 0 and "unknown option" in out
+
+
+@pytest.mark.gpu
+def test_plugins_against_oracle_on_gpu():
+    print("[case] C++ plug-ins through TimedAlgorithm", flush=True)
+    rc, out = run([TEST_BIN, "gpu"])
+    assert rc == 0, out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("alg,extra", [
+    ("gol-hip-temporal-64", ["--temporal-steps=8"]),
+    ("gol-hip-local-64", []),
+    ("gol-hip-temporal-32", ["--base-grid-encoding=int"]),
+])
+def test_cli_validates(alg, extra):
+    print("[case] stencils", alg, flush=True)
+    rc, out = run([STENCILS, f"--algorithm={alg}", "--grid-dimensions-x=1024", "--grid-dimensions-y=768",
+                   "--iterations=300", "--validate", "--measurement-rounds=2", "--warmup-rounds=1",
+                   "--measure-speedup", "--colorful=false"] + extra)
+    assert rc == 0, out
+    assert "Validation successful" in out
+    assert out.count("Time report:") == 2
+    assert "performed iters:  300" in out
+
+
+@pytest.mark.gpu
+def test_cli_lexicon_and_time_budget():
+    print("[case] stencils lexicon", flush=True)
+    rc, out = run([STENCILS, "--algorithm=gol-hip-rows-64", "--grid-dimensions-x=256", "--grid-dimensions-y=256",
+                   "--iterations=120", "--data-loader=lexicon",
+                   "--pattern-expression=gosper-glider-gun[10, 10]; glider[200, 200]", "--validate",
+                   "--validation-algorithm=gol-hip-temporal-64", "--max-runtime-seconds=100"])
+    assert rc == 0, out
+    assert "Validation successful" in out
