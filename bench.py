@@ -259,6 +259,8 @@ def main() -> int:
     ap.add_argument("--lane-dwords", type=int, default=0)
     ap.add_argument("--band-rows", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--no-persistent", action="store_true",
+                    help="one launch per block of generations instead of the persistent slab launch")
     ap.add_argument("--kernel", default="temporal", choices=["temporal", "plain"])
     ap.add_argument("--cpu-sample-generations", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -284,10 +286,11 @@ def main() -> int:
     src.upload(words)
     src.sync()
     grid = golhip.Grid(X, Y, word_bits=64, kernel=kernel, temporal_steps=args.temporal_steps,
-                       lane_dwords=args.lane_dwords, band_rows=args.band_rows, waves=args.waves, device=0)
+                       lane_dwords=args.lane_dwords, band_rows=args.band_rows, waves=args.waves, device=0,
+                       flags=golhip.FLAG_NO_PERSISTENT if args.no_persistent else 0)
     info = grid.info()
     log(f"kernel {info.kernel} lane_dwords {info.lane_dwords} band_rows {info.band_rows} waves "
-        f"{info.waves} temporal_steps {info.temporal_steps} tiles {info.tiles}")
+        f"{info.waves} temporal_steps {info.temporal_steps} tiles {info.tiles} persistent {info.persistent}")
 
     def one_step() -> float:
         grid.upload_device(src.device_state())
@@ -332,7 +335,8 @@ def main() -> int:
     # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes
     # (calibrated against a copy kernel), valid for the profiled shape and depth
     traffic = None
-    prof = os.path.join(ROOT, "profiles", "tile_kernel_16384x16384_random.md")
+    prof = os.path.join(ROOT, "profiles", "slab_kernel_16384x16384_random.md" if info.persistent
+                        else "tile_kernel_16384x16384_random.md")
     if (os.path.exists(prof) and (info.lane_dwords, info.band_rows, info.waves, info.temporal_steps) == (8, 10, 8, 8)
             and args.kernel == "temporal"):
         import re
@@ -358,6 +362,7 @@ def main() -> int:
             "kernel": args.kernel,
             "lane_dwords": info.lane_dwords, "band_rows": info.band_rows, "waves": info.waves,
             "temporal_steps": info.temporal_steps, "tiles_per_launch": info.tiles,
+            "persistent": bool(info.persistent),
             "launches_per_step": launches // args.steps,
             "bit_exact_vs_oracle": exact,
         },

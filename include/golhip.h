@@ -52,7 +52,9 @@ enum gol_kernel {
     GOL_KERNEL_AUTO = 0,     /* temporal tiles, shape and depth from the built-in table */
     GOL_KERNEL_PLAIN = 1,    /* one generation per launch
                                 (reference: cuda-naive-bitwise/cuda_naive_bitwise_kernel.cu:32-79) */
-    GOL_KERNEL_TEMPORAL = 2, /* T generations per launch on LDS-exchanged register tiles
+    GOL_KERNEL_TEMPORAL = 2, /* blocks of T generations on LDS-exchanged register tiles; when one
+                                tile spans the grid's width and all tiles fit on the device at
+                                once, a single persistent launch runs every block
                                 (reference: cuda-temporal-rowed/temporal_rowed.cu:33-229) */
     GOL_KERNEL_ACTIVE = 3    /* change tracking over a device-side active-tile list: a tile is
                                 advanced only when it or a neighbour changed over the last block
@@ -72,16 +74,22 @@ typedef struct gol_config {
     uint32_t band_rows;      /* rows per wavefront band; 0 = chosen by the library */
     uint32_t waves;          /* wavefronts per workgroup; 0 = chosen by the library */
     int32_t device;          /* HIP device ordinal; -1 = current device */
-    uint32_t reserved;
+    uint32_t flags;          /* GOL_FLAG_* */
 } gol_config;
+
+/* gol_config.flags */
+#define GOL_FLAG_NO_PERSISTENT 1u /* temporal kernel: one launch per block of generations, never the
+                                     persistent slab launch */
 
 typedef struct gol_info {
     uint32_t kernel;          /* resolved enum gol_kernel */
     uint32_t lane_dwords;     /* resolved tile shape */
     uint32_t band_rows;
     uint32_t waves;
-    uint32_t temporal_steps;  /* generations per launch */
+    uint32_t temporal_steps;  /* generations per launch (per block for the persistent launch) */
     uint32_t tiles;           /* workgroups per launch */
+    uint32_t persistent;      /* 1: one cooperative launch per gol_step, slabs exchange halos in memory */
+    uint32_t reserved;
     uint64_t launches;        /* kernel launches issued so far */
     uint64_t generations;     /* generations enqueued so far (actually_performed_iterations) */
     uint64_t active_tiles_total; /* GOL_KERNEL_ACTIVE: tiles computed, summed over blocks (syncs) */

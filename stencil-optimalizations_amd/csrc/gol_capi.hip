@@ -137,6 +137,13 @@ struct gol_handle {
     int n_tiles = 0;
     uint64_t blocks = 0;          // change-tracking blocks run
     bool flags_valid = false;     // false: the next block treats every tile as active
+    // persistent slab launch (GOL_KERNEL_TEMPORAL, one tile spans the width)
+    bool persistent = false;
+    uint32_t* xbuf = nullptr;             // [2][tiles_y][2][steps][x_dw]
+    unsigned long long* slab_flags = nullptr;  // [tiles_y]
+    unsigned int* slab_error = nullptr;
+    unsigned long long slab_epoch = 0;
+    bool slab_check = false;              // a slab launch ran since the error flag was last read
 };
 
 namespace {
@@ -201,6 +208,64 @@ int plan(gol_handle* h) {
         return fail(GOL_ERR_RUNTIME, "temporal_steps %d does not fit the tile: %s", steps, why);
     h->steps = steps;
     h->n_tiles = h->geom.tiles_x * h->geom.tiles_y;
+    // persistent slabs: every tile resident at once, one tile across the width,
+    // interiors at least as tall as the exchanged halos
+    if (h->kernel == GOL_KERNEL_TEMPORAL && !(c.flags & GOL_FLAG_NO_PERSISTENT) && !h->one_tile &&
+        h->geom.tiles_x == 1 && h->geom.tiles_y > 1 && h->geom.stride_rows >= steps && steps <= 8 &&
+        h->x_dw <= 64 * s.lane_dwords) {
+        int capacity = 0;
+        if (slab_capacity(s, &capacity) && capacity >= h->geom.tiles_y) h->persistent = true;
+    }
+    return GOL_OK;
+}
+
+// Checks the slab error flag (a neighbour wait that ran out); waits for the stream.
+int check_slab_error(gol_handle* h) {
+    if (!h->slab_check) return GOL_OK;
+    unsigned int err = 0;
+    hipError_t e = hipMemcpyAsync(&err, h->slab_error, sizeof(err), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail_hip(e, "slab error flag");
+    h->slab_check = false;
+    if (err) {
+        (void)hipMemsetAsync(h->slab_error, 0, sizeof(unsigned int), h->stream);
+        return fail(GOL_ERR_RUNTIME, "persistent launch stopped: a slab waited too long for its neighbours; "
+                    "the state is undefined (set GOL_FLAG_NO_PERSISTENT to avoid the persistent launch)");
+    }
+    return GOL_OK;
+}
+
+// One cooperative launch per at most 65536 blocks: every slab stays resident
+// and exchanges halos with its neighbours between blocks.
+int enqueue_persistent(gol_handle* h, uint64_t generations) {
+    const uint64_t T = static_cast<uint64_t>(h->steps);
+    uint64_t left = generations;
+    while (left > 0) {
+        const uint64_t blocks = std::min<uint64_t>(left / T, 65536);
+        const uint64_t last = blocks < 65536 ? left - blocks * T : 0;
+        SlabArgs a;
+        a.src = h->buf[h->cur];
+        a.dst = h->buf[h->cur ^ 1];
+        a.geo = to_device_geo(h->geom, h->x_dw, h->y_rows);
+        a.steps = h->steps;
+        a.blocks = static_cast<int>(blocks);
+        a.last_gens = static_cast<int>(last);
+        a.tiles_y = h->geom.tiles_y;
+        a.xbuf = h->xbuf;
+        a.flags = h->slab_flags;
+        a.epoch = h->slab_epoch;
+        a.error = h->slab_error;
+        hipError_t e = launch_slab(h->shape, a, h->stream);
+        if (e != hipSuccess) return fail_hip(e, "slab_kernel launch");
+        const uint64_t total = blocks + (last ? 1 : 0);
+        h->slab_epoch += total > 0 ? total - 1 : 0;  // exchanges happen between blocks
+        h->cur ^= 1;
+        h->launches += 1;
+        const uint64_t done = blocks * T + last;
+        h->generations += done;
+        left -= done;
+        h->slab_check = true;
+    }
     return GOL_OK;
 }
 
@@ -238,6 +303,7 @@ int enqueue_active_block(gol_handle* h) {
 }
 
 int enqueue(gol_handle* h, uint64_t generations) {
+    if (h->persistent) return enqueue_persistent(h, generations);
     uint64_t left = generations;
     if (h->kernel == GOL_KERNEL_ACTIVE) {
         while (left >= static_cast<uint64_t>(h->steps)) {
@@ -308,12 +374,12 @@ int gol_create(const gol_config* config, gol_handle** out) {
     h->x_dw = static_cast<int64_t>(config->x_cells / 32);
     h->y_rows = static_cast<int64_t>(config->y_cells);
     h->n_dwords = static_cast<size_t>(h->x_dw) * static_cast<size_t>(h->y_rows);
+    DeviceGuard g(dev);
     rc = plan(h);
     if (rc) {
         delete h;
         return rc;
     }
-    DeviceGuard g(dev);
     const size_t bytes = h->n_dwords * sizeof(uint32_t);
     if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreate(&h->ev_start)) != hipSuccess || (e = hipEventCreate(&h->ev_stop)) != hipSuccess ||
@@ -332,6 +398,19 @@ int gol_create(const gol_config* config, gol_handle** out) {
             (e = hipMemsetAsync(h->counts, 0, 2 * sizeof(int), h->stream)) != hipSuccess ||
             (e = hipMemsetAsync(h->active_total, 0, sizeof(unsigned long long), h->stream)) != hipSuccess) {
             rc = fail_hip(e, "gol_create (change tracking state)");
+            gol_destroy(h);
+            return rc;
+        }
+    }
+    if (h->persistent) {
+        const size_t xdw = 2ull * h->geom.tiles_y * 2 * h->steps * static_cast<size_t>(h->x_dw);
+        if ((e = hipMalloc(&h->xbuf, xdw * 4)) != hipSuccess ||
+            (e = hipMalloc(&h->slab_flags, h->geom.tiles_y * sizeof(unsigned long long))) != hipSuccess ||
+            (e = hipMalloc(&h->slab_error, sizeof(unsigned int))) != hipSuccess ||
+            (e = hipMemsetAsync(h->slab_flags, 0, h->geom.tiles_y * sizeof(unsigned long long), h->stream)) !=
+                hipSuccess ||
+            (e = hipMemsetAsync(h->slab_error, 0, sizeof(unsigned int), h->stream)) != hipSuccess) {
+            rc = fail_hip(e, "gol_create (persistent slab state)");
             gol_destroy(h);
             return rc;
         }
@@ -387,6 +466,7 @@ int gol_step_timed(gol_handle* h, uint64_t generations, double* gpu_ms) {
     if (rc) return rc;
     if ((e = hipEventRecord(h->ev_stop, h->stream)) != hipSuccess) return fail_hip(e, "hipEventRecord");
     if ((e = hipEventSynchronize(h->ev_stop)) != hipSuccess) return fail_hip(e, "hipEventSynchronize");
+    if ((rc = check_slab_error(h))) return rc;
     float ms = 0.f;
     if ((e = hipEventElapsedTime(&ms, h->ev_start, h->ev_stop)) != hipSuccess)
         return fail_hip(e, "hipEventElapsedTime");
@@ -400,6 +480,8 @@ int gol_download(gol_handle* h, void* words, size_t n_words) {
     const size_t want = h->n_dwords * 32 / h->cfg.word_bits;
     if (n_words != want) return fail(GOL_ERR_INVALID_ARGUMENT, "expected %zu words, got %zu", want, n_words);
     DeviceGuard g(h->device);
+    int rc = check_slab_error(h);
+    if (rc) return rc;
     hipError_t e = hipMemcpyAsync(words, h->buf[h->cur], h->n_dwords * 4, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return fail_hip(e, "gol_download");
@@ -412,7 +494,7 @@ int gol_sync(gol_handle* h) {
     DeviceGuard g(h->device);
     hipError_t e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return fail_hip(e, "gol_sync");
-    return GOL_OK;
+    return check_slab_error(h);
 }
 
 int gol_device_state(gol_handle* h, void** device_words) {
@@ -443,6 +525,7 @@ int gol_get_info(gol_handle* h, gol_info* info) {
         info->waves = static_cast<uint32_t>(h->shape.waves);
         info->temporal_steps = static_cast<uint32_t>(h->steps);
         info->tiles = static_cast<uint32_t>(h->geom.tiles_x * h->geom.tiles_y);
+        info->persistent = h->persistent ? 1u : 0u;
     }
     info->launches = h->launches;
     info->generations = h->generations;
@@ -471,6 +554,9 @@ void gol_destroy(gol_handle* h) {
         if (h->lists) (void)hipFree(h->lists);
         if (h->counts) (void)hipFree(h->counts);
         if (h->active_total) (void)hipFree(h->active_total);
+        if (h->xbuf) (void)hipFree(h->xbuf);
+        if (h->slab_flags) (void)hipFree(h->slab_flags);
+        if (h->slab_error) (void)hipFree(h->slab_error);
         if (h->ev_start) (void)hipEventDestroy(h->ev_start);
         if (h->ev_stop) (void)hipEventDestroy(h->ev_stop);
         if (h->stream) (void)hipStreamDestroy(h->stream);

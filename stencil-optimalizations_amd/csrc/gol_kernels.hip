@@ -310,6 +310,183 @@ __global__ __launch_bounds__(64 * NW) void item_kernel(const BatchGrid* __restri
     process_tile<DW, BR, NW, false>(src, dst, g.geo, it.tile, it.gens, lds);
 }
 
+// ---------------------------------------------------------------------------
+// Persistent slab kernel: one cooperative launch advances a grid whose rows a
+// single tile spans (tiles_x == 1) by many blocks of T generations. Workgroup
+// ty keeps its tile (interior rows plus T halo rows above and below) in VGPRs
+// for the whole launch. After each block it exports its first and last T
+// interior rows to an exchange buffer, publishes the block count, waits for
+// the two neighbouring slabs to publish theirs and imports their rows as its
+// new halos. The tile load and store happen once per launch instead of once
+// per block. Waits are bounded: a slab that waits too long raises *error and
+// every slab stops, so the launch always ends.
+// ---------------------------------------------------------------------------
+constexpr int kSlabSpinLimit = 1 << 20;
+// Deepest halo (generations per block) a slab launch takes.
+constexpr int kSlabMaxSteps = 8;
+
+// Relaxed polling (no cache invalidation per iteration); the caller issues one
+// acquire fence after the wait succeeds.
+__device__ bool wait_counter(const unsigned long long* p, unsigned long long target, unsigned int* error) {
+    for (int i = 0; i < kSlabSpinLimit; ++i) {
+        if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
+        if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return false;
+        __builtin_amdgcn_s_sleep(2);
+    }
+    __hip_atomic_fetch_or(error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return false;
+}
+
+// The block loop of one slab; the row and column masks are fixed for the
+// slab, so each instantiation holds one copy of the band update.
+template <int DW, int BR, int NW, int STAGE_ROWS, bool MASK_ROWS, bool MASK_COLS>
+__device__ __forceinline__ bool slab_blocks(const SlabArgs& a, uint32_t (&s)[BR][DW], EdgeRows<DW, NW>& lds,
+                                            uint32_t* halo_stage, int* keep_going, int lane, int w, int gx0,
+                                            int row_lo, int row_hi, const uint32_t (&colmask)[DW]) {
+    const TileGeo& g = a.geo;
+    const int ty = blockIdx.x;
+    const int T = a.steps;
+    const int tile_rows = NW * BR;
+    const bool vec_ok = (g.x_dw % DW) == 0;
+    const size_t plane = static_cast<size_t>(T) * g.x_dw;  // T rows of one side
+    const int total_blocks = a.blocks + (a.last_gens > 0 ? 1 : 0);
+    for (int b = 0; b < total_blocks; ++b) {
+        const int gens = b < a.blocks ? T : a.last_gens;
+        run_generations<DW, BR, NW, MASK_ROWS, MASK_COLS>(s, lds, gens, lane, w, row_lo, row_hi, colmask);
+        if (b + 1 == total_blocks) break;
+
+        const int par = b & 1;
+        uint32_t* mine = a.xbuf + (static_cast<size_t>(par) * a.tiles_y + ty) * 2 * plane;
+#pragma unroll
+        for (int r = 0; r < BR; ++r) {
+            const int tr = w * BR + r;
+            if (tr >= T && tr < 2 * T)
+                store_seg<DW>(mine + static_cast<size_t>(tr - T) * g.x_dw, g.x_dw, 0, gx0, vec_ok, s[r]);
+            if (tr >= tile_rows - 2 * T && tr < tile_rows - T)
+                store_seg<DW>(mine + plane + static_cast<size_t>(tr - (tile_rows - 2 * T)) * g.x_dw, g.x_dw, 0, gx0,
+                              vec_ok, s[r]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // Publish: every wave's stores have left it (vmcnt 0) before the
+        // barrier; then one thread writes back the L2 (release fence), waits
+        // for the write-back to finish (the fence alone leaves no wait between
+        // the write-back and the next store) and only then bumps the counter.
+        // Subscribe: one thread polls both neighbours, then
+        // invalidates the CU's L1 and the L2 once (acquire fence) before any
+        // wave of the workgroup reads their rows.
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const unsigned long long target = a.epoch + static_cast<unsigned long long>(b) + 1;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __builtin_amdgcn_s_waitcnt(0);
+            __hip_atomic_store(&a.flags[ty], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            bool ok = true;
+            if (ty > 0) ok = wait_counter(&a.flags[ty - 1], target, a.error);
+            if (ok && ty + 1 < a.tiles_y) ok = wait_counter(&a.flags[ty + 1], target, a.error);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            *keep_going = ok ? 1 : 0;
+        }
+        __syncthreads();
+        if (!*keep_going) return false;  // the whole workgroup leaves together; the error is recorded
+
+        // Import: all threads copy the neighbours' rows into LDS with wide
+        // independent loads (their latencies overlap), then each wavefront
+        // takes its halo rows from LDS. Both sides go in one pass when the
+        // stage holds 2T rows, else the upper side first, then the lower.
+        const uint32_t* above =
+            a.xbuf + (static_cast<size_t>(par) * a.tiles_y + (ty > 0 ? ty - 1 : 0)) * 2 * plane + plane;
+        const uint32_t* below =
+            a.xbuf + (static_cast<size_t>(par) * a.tiles_y + (ty + 1 < a.tiles_y ? ty + 1 : ty)) * 2 * plane;
+        const int passes = 2 * T <= STAGE_ROWS ? 1 : 2;
+        const size_t below_at = passes == 1 ? plane : 0;  // stage offset of the lower side
+#pragma nounroll
+        for (int pass = 0; pass < passes; ++pass) {
+            const bool take_above = (passes == 1 || pass == 0) && ty > 0;
+            const bool take_below = (passes == 1 || pass == 1) && ty + 1 < a.tiles_y;
+            if ((plane & 3) == 0) {  // xbuf planes start on 16-byte boundaries
+                const int quads = static_cast<int>(plane >> 2);
+                uint4* stage = reinterpret_cast<uint4*>(halo_stage);
+                for (int i = threadIdx.x; i < quads; i += 64 * NW) {
+                    if (take_above) stage[i] = reinterpret_cast<const uint4*>(above)[i];
+                    if (take_below) stage[(below_at >> 2) + i] = reinterpret_cast<const uint4*>(below)[i];
+                }
+            } else {
+                for (int i = threadIdx.x; i < static_cast<int>(plane); i += 64 * NW) {
+                    if (take_above) halo_stage[i] = above[i];
+                    if (take_below) halo_stage[below_at + i] = below[i];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < BR; ++r) {
+                const int tr = w * BR + r;
+                const bool top = take_above && tr < T;
+                const bool bottom = take_below && tr >= tile_rows - T;
+                if (top || bottom) {
+                    const uint32_t* row =
+                        halo_stage + (top ? static_cast<size_t>(tr) * g.x_dw
+                                          : below_at + static_cast<size_t>(tr - (tile_rows - T)) * g.x_dw);
+#pragma unroll
+                    for (int d = 0; d < DW; ++d) s[r][d] = gx0 + d < g.x_dw ? row[gx0 + d] : 0u;
+                }
+            }
+            __syncthreads();  // the stage is refilled by the next pass or exchange
+        }
+    }
+    return true;
+}
+
+// Rows of 64*DW dwords staged per import: both sides of the deepest halo
+// (2 x kSlabMaxSteps) when that fits in LDS beside the edge rows, else one side.
+template <int DW, int NW>
+constexpr int slab_stage_rows() {
+    return sizeof(EdgeRows<DW, NW>) + 2 * kSlabMaxSteps * 64 * DW * sizeof(uint32_t) <= 156 * 1024
+               ? 2 * kSlabMaxSteps
+               : kSlabMaxSteps;
+}
+
+template <int DW, int BR, int NW, bool MASK_COLS>
+__global__ __launch_bounds__(64 * NW) void slab_kernel(SlabArgs a) {
+    constexpr int STAGE_ROWS = slab_stage_rows<DW, NW>();
+    __shared__ EdgeRows<DW, NW> lds;
+    __shared__ uint32_t halo_stage[STAGE_ROWS * 64 * DW];
+    __shared__ int keep_going;
+    const TileGeo& g = a.geo;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+    const int ty = blockIdx.x;
+    const int tile_rows = NW * BR;
+    const int y0 = ty * g.stride_rows - g.halo_rows;
+    const int gx0 = lane * DW;
+    const int gy0 = y0 + w * BR;
+    const bool vec_ok = (g.x_dw % DW) == 0;
+
+    uint32_t s[BR][DW];
+#pragma unroll
+    for (int r = 0; r < BR; ++r) load_seg<DW>(a.src, g.x_dw, g.y_rows, gy0 + r, gx0, vec_ok, s[r]);
+    clear_edge_slots<DW, NW>(lds, lane, w);
+
+    uint32_t colmask[DW];
+#pragma unroll
+    for (int d = 0; d < DW; ++d) colmask[d] = (gx0 + d < g.x_dw) ? ~0u : 0u;
+    const int row_lo = gy0 < 0 ? -gy0 : 0;
+    const int row_hi = g.y_rows - gy0 < BR ? g.y_rows - gy0 : BR;
+    const bool edge_y = y0 < 0 || y0 + tile_rows > g.y_rows;  // workgroup-uniform
+    const bool done = edge_y ? slab_blocks<DW, BR, NW, STAGE_ROWS, true, MASK_COLS>(a, s, lds, halo_stage, &keep_going,
+                                                                                     lane, w, gx0, row_lo, row_hi, colmask)
+                             : slab_blocks<DW, BR, NW, STAGE_ROWS, false, MASK_COLS>(
+                                   a, s, lds, halo_stage, &keep_going, lane, w, gx0, row_lo, row_hi, colmask);
+    if (!done) return;
+#pragma unroll
+    for (int r = 0; r < BR; ++r) {
+        const int tr = w * BR + r;
+        const int gy = gy0 + r;
+        if (tr >= g.halo_rows && tr < tile_rows - g.halo_rows && gy >= 0 && gy < g.y_rows)
+            store_seg<DW>(a.dst, g.x_dw, gy, gx0, vec_ok, s[r]);
+    }
+}
+
 // Active-tile list for the next block: tile t is active when it or one of its
 // eight neighbours changed in the last block. Also clears the flags the next
 // block writes and the counter of the list after next.
@@ -454,6 +631,45 @@ hipError_t launch_items(const TileShape& shape, const BatchGrid* grids, const Wo
     GOLHIP_FOR_EACH_SHAPE(GOLHIP_LAUNCH)
 #undef GOLHIP_LAUNCH
     return hipErrorInvalidValue;
+}
+
+// The slab kernels compiled: full-width lanes of 8 dwords (rows of up to 16384 cells).
+#define GOLHIP_FOR_EACH_SLAB_SHAPE(X) \
+    X(8, 10, 8)                        \
+    X(8, 5, 16)                        \
+    X(8, 4, 16)                        \
+    X(8, 8, 8)
+
+static const void* slab_function(const TileShape& shape, bool mask_cols) {
+    const void* f = nullptr;
+#define GOLHIP_SLAB_FN(DW, BR, NW)                                                                   \
+    if (shape.lane_dwords == DW && shape.band_rows == BR && shape.waves == NW)                       \
+        f = mask_cols ? reinterpret_cast<const void*>(&slab_kernel<DW, BR, NW, true>)                \
+                      : reinterpret_cast<const void*>(&slab_kernel<DW, BR, NW, false>);
+    GOLHIP_FOR_EACH_SLAB_SHAPE(GOLHIP_SLAB_FN)
+#undef GOLHIP_SLAB_FN
+    return f;
+}
+
+bool slab_capacity(const TileShape& shape, int* workgroups) {
+    const void* f = slab_function(shape, false);
+    if (!f) return false;
+    int dev = 0, cus = 0, coop = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return false;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop) return false;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 64 * shape.waves, 0) != hipSuccess) return false;
+    *workgroups = per_cu * cus;
+    return true;
+}
+
+hipError_t launch_slab(const TileShape& shape, const SlabArgs& args, hipStream_t stream) {
+    const void* f = slab_function(shape, 64 * shape.lane_dwords > args.geo.x_dw);
+    if (!f) return hipErrorInvalidValue;
+    SlabArgs copy = args;
+    void* kargs[] = {&copy};
+    return hipLaunchCooperativeKernel(f, dim3(static_cast<unsigned>(args.tiles_y)), dim3(64 * shape.waves), kargs, 0,
+                                      stream);
 }
 
 hipError_t launch_plain(int lane_dwords, const uint32_t* src, uint32_t* dst, int64_t x_dw, int64_t y_rows,

@@ -77,6 +77,26 @@ hipError_t launch_build_active_list(const uint32_t* changed_prev, uint32_t* chan
 hipError_t launch_items(const TileShape& shape, const BatchGrid* grids, const WorkItem* items, int n_items,
                         hipStream_t stream);
 
+// Persistent slab launch over a grid that one tile spans in x (tiles_x == 1).
+struct SlabArgs {
+    const uint32_t* src;
+    uint32_t* dst;
+    TileGeo geo;                // halo_rows == steps
+    int steps;                  // T: generations per block (the halo depth), at most kSlabMaxSteps (8)
+    int blocks;                 // full blocks of T generations
+    int last_gens;              // generations of a final shorter block (0: none)
+    int tiles_y;                // workgroups
+    uint32_t* xbuf;             // [2][tiles_y][2][T][x_dw] exchanged halo rows
+    unsigned long long* flags;  // [tiles_y] blocks published so far, offset by epoch
+    unsigned long long epoch;   // flag value every slab holds before this launch
+    unsigned int* error;        // raised when a neighbour wait runs out
+};
+
+// Workgroups of the slab kernel that can be resident at once (false if the
+// device cannot launch cooperatively).
+bool slab_capacity(const TileShape& shape, int* workgroups);
+hipError_t launch_slab(const TileShape& shape, const SlabArgs& args, hipStream_t stream);
+
 // Plain kernel: one generation per launch, global memory to global memory.
 // lane_dwords in {1,2,4,8}.
 hipError_t launch_plain(int lane_dwords, const uint32_t* src, uint32_t* dst, int64_t x_dw, int64_t y_rows,

@@ -62,8 +62,11 @@ class GolConfig(ctypes.Structure):
         ("band_rows", ctypes.c_uint32),
         ("waves", ctypes.c_uint32),
         ("device", ctypes.c_int32),
-        ("reserved", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
     ]
+
+
+FLAG_NO_PERSISTENT = 1
 
 
 class GolInfo(ctypes.Structure):
@@ -74,6 +77,8 @@ class GolInfo(ctypes.Structure):
         ("waves", ctypes.c_uint32),
         ("temporal_steps", ctypes.c_uint32),
         ("tiles", ctypes.c_uint32),
+        ("persistent", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
         ("launches", ctypes.c_uint64),
         ("generations", ctypes.c_uint64),
         ("active_tiles_total", ctypes.c_uint64),
@@ -204,10 +209,10 @@ class Grid:
 
     def __init__(self, x_cells: int, y_cells: int, word_bits: int = 64, kernel: int = KERNEL_AUTO,
                  temporal_steps: int = 0, lane_dwords: int = 0, band_rows: int = 0, waves: int = 0,
-                 device: int = -1):
+                 device: int = -1, flags: int = 0):
         self.lib = load_library()
         cfg = GolConfig(x_cells, y_cells, word_bits, kernel, temporal_steps, lane_dwords, band_rows,
-                        waves, device, 0)
+                        waves, device, flags)
         h = ctypes.c_void_p()
         _check(self.lib.gol_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h

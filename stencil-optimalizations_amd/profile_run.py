@@ -1,7 +1,9 @@
 """Fixed workloads to run under rocprofv3 (the profiles under profiles/ come from these).
 
+  python3 profile_run.py slab     16384^2 random, 3 persistent slab launches of 10000 generations
+                                  (the bench's configuration: one launch per step)
   python3 profile_run.py tile     16384^2 random, 800 generations with the temporal-tile kernel
-                                  (100 launches of 8 generations, the bench's configuration)
+                                  (100 launches of 8 generations, persistent launch switched off)
   python3 profile_run.py plain    16384^2 random, 100 generations with the plain kernel
   python3 profile_run.py active   16384^2 mostly-stable patches, 2400 generations, change tracking
   python3 profile_run.py batch    256 mixed grids (sides 64..2048, 100..500 generations), one batch run
@@ -29,11 +31,22 @@ def main(what: str) -> None:
         w = golhip.fill_random(16384, 16384, 42)
         kernel = golhip.KERNEL_TEMPORAL if what == "tile" else golhip.KERNEL_PLAIN
         gens = 800 if what == "tile" else 100
-        with golhip.Grid(16384, 16384, kernel=kernel, device=0) as g:
+        with golhip.Grid(16384, 16384, kernel=kernel, device=0, flags=golhip.FLAG_NO_PERSISTENT) as g:
             g.upload(w)
             ms = g.step_timed(gens)
             info = g.info()
         print(f"{what}: {gens} generations, {info.launches} launches, {ms:.2f} ms GPU", flush=True)
+    elif what == "slab":
+        w = golhip.fill_random(16384, 16384, 42)
+        with golhip.Grid(16384, 16384, kernel=golhip.KERNEL_TEMPORAL, device=0) as g:
+            if not g.info().persistent:
+                raise SystemExit("slab: the persistent launch is not available on this device")
+            for _ in range(3):
+                g.upload(w)
+                ms = g.step_timed(10000)
+                print(f"slab: 10000 generations, {ms:.2f} ms GPU", flush=True)
+            info = g.info()
+        print(f"slab: {info.launches} launches", flush=True)
     elif what == "active":
         w = golhip.fill_patches(16384, 16384, 42, 256, 64, 0.35)
         with golhip.Grid(16384, 16384, kernel=golhip.KERNEL_ACTIVE, device=0) as g:

@@ -167,3 +167,64 @@ def test_wide_grid_x_tiles():
         note("wide", x, y, steps)
         out = run_gpu(w, x, y, gens, kernel=golhip.KERNEL_TEMPORAL, temporal_steps=steps)
         assert np.array_equal(ol.unpack(out, x, y), ref)
+
+
+@pytest.mark.parametrize("x,y,gens,steps", [
+    (16384, 2000, 37, 8),     # full-width slabs, ragged height, partial last block
+    (16384, 700, 64, 4),
+    (8192, 1500, 45, 6),      # narrower than the lanes: columns masked
+    (12288, 333, 20, 2),
+    (16384, 96, 9, 8),        # fewer generations than one block
+])
+def test_persistent_slabs_against_oracle(x, y, gens, steps):
+    note("persistent", x, y, gens, steps)
+    w = ol.pack(ol.bernoulli_cells(x, y, x + y + gens, 0.45))
+    ref = ol.bitpar_run(w, x, y, gens)
+    with golhip.Grid(x, y, kernel=golhip.KERNEL_TEMPORAL, temporal_steps=steps, lane_dwords=8, band_rows=10,
+                     waves=8) as g:
+        persistent = g.info().persistent
+        g.upload(w)
+        g.step(gens)
+        out = g.download()
+    assert np.array_equal(out, ref)
+    with golhip.Grid(x, y, kernel=golhip.KERNEL_TEMPORAL, temporal_steps=steps, lane_dwords=8, band_rows=10,
+                     waves=8, flags=golhip.FLAG_NO_PERSISTENT) as g:
+        assert g.info().persistent == 0
+        g.upload(w)
+        g.step(gens)
+        assert np.array_equal(g.download(), ref)
+    assert persistent in (0, 1)
+
+
+def test_persistent_repeated_launches_match_fixture():
+    """Ten persistent launches of 10000 generations on one handle, the input
+    coming alternately from a device buffer and from the host (the benchmark's
+    sequence): every result equals the fixture. A halo row published before
+    its write-back reached memory would make one of them differ."""
+    w = golhip.fill_random(16384, 16384, 42)
+    f = [r for r in FIX["bitpar_runs"] if r["name"] == "random-16384" and r["generations"] == 10000][0]
+    with golhip.Grid(16384, 16384, kernel=golhip.KERNEL_PLAIN) as src, golhip.Grid(16384, 16384) as g:
+        assert g.info().persistent == 1
+        src.upload(w)
+        src.sync()
+        for i in range(10):
+            note("persistent repeat", i)
+            if i % 2:
+                g.upload(w)
+            else:
+                g.upload_device(src.device_state())
+            g.step(10000)
+            assert ol.sha256(g.download()) == f["sha256"], f"launch {i}"
+
+
+def test_persistent_used_for_16384_and_split_calls():
+    w = golhip.fill_random(16384, 16384, 42)
+    f = [r for r in FIX["bitpar_runs"] if r["name"] == "random-16384" and r["generations"] == 1000][0]
+    with golhip.Grid(16384, 16384) as g:
+        info = g.info()
+        note("persistent default", info.persistent, info.lane_dwords, info.band_rows, info.waves, info.temporal_steps)
+        assert info.persistent == 1
+        g.upload(w)
+        for k in (1, 7, 8, 100, 300, 584):   # 1000 generations in uneven calls
+            g.step(k)
+        assert ol.sha256(g.download()) == f["sha256"]
