@@ -139,7 +139,7 @@ struct gol_handle {
     bool flags_valid = false;     // false: the next block treats every tile as active
     // persistent slab launch (GOL_KERNEL_TEMPORAL, one tile spans the width)
     bool persistent = false;
-    uint32_t* xbuf = nullptr;             // [2][tiles_y][2][steps][x_dw]
+    unsigned long long* xbuf = nullptr;   // [2][tiles_y][2][steps][32 * lane_dwords] dword pairs
     unsigned long long* slab_flags = nullptr;  // [tiles_y]
     unsigned int* slab_error = nullptr;
     unsigned long long slab_epoch = 0;
@@ -212,9 +212,10 @@ int plan(gol_handle* h) {
     // interiors at least as tall as the exchanged halos
     if (h->kernel == GOL_KERNEL_TEMPORAL && !(c.flags & GOL_FLAG_NO_PERSISTENT) && !h->one_tile &&
         h->geom.tiles_x == 1 && h->geom.tiles_y > 1 && h->geom.stride_rows >= steps && steps <= 8 &&
-        h->x_dw <= 64 * s.lane_dwords) {
+        h->x_dw <= 64 * s.lane_dwords && s.lane_dwords % 2 == 0) {
         int capacity = 0;
-        if (slab_capacity(s, &capacity) && capacity >= h->geom.tiles_y) h->persistent = true;
+        if (slab_capacity(s, 64 * s.lane_dwords > h->x_dw, &capacity) && capacity >= h->geom.tiles_y)
+            h->persistent = true;
     }
     return GOL_OK;
 }
@@ -403,8 +404,8 @@ int gol_create(const gol_config* config, gol_handle** out) {
         }
     }
     if (h->persistent) {
-        const size_t xdw = 2ull * h->geom.tiles_y * 2 * h->steps * static_cast<size_t>(h->x_dw);
-        if ((e = hipMalloc(&h->xbuf, xdw * 4)) != hipSuccess ||
+        const size_t pairs = 2ull * h->geom.tiles_y * 2 * h->steps * 32 * static_cast<size_t>(h->shape.lane_dwords);
+        if ((e = hipMalloc(&h->xbuf, pairs * sizeof(unsigned long long))) != hipSuccess ||
             (e = hipMalloc(&h->slab_flags, h->geom.tiles_y * sizeof(unsigned long long))) != hipSuccess ||
             (e = hipMalloc(&h->slab_error, sizeof(unsigned int))) != hipSuccess ||
             (e = hipMemsetAsync(h->slab_flags, 0, h->geom.tiles_y * sizeof(unsigned long long), h->stream)) !=

@@ -318,15 +318,18 @@ __global__ __launch_bounds__(64 * NW) void item_kernel(const BatchGrid* __restri
 // interior rows to an exchange buffer, publishes the block count, waits for
 // the two neighbouring slabs to publish theirs and imports their rows as its
 // new halos. The tile load and store happen once per launch instead of once
-// per block. Waits are bounded: a slab that waits too long raises *error and
-// every slab stops, so the launch always ends.
+// per block. Two exchange buffers alternate by block parity: a slab rewrites
+// its rows of block b only at block b + 2, after both neighbours published
+// block b + 1, which they do only after importing its block b rows. The
+// counters carry an epoch across launches and are never reset. Waits are
+// bounded: a slab that waits too long raises *error and every slab stops, so
+// the launch always ends.
 // ---------------------------------------------------------------------------
 constexpr int kSlabSpinLimit = 1 << 20;
 // Deepest halo (generations per block) a slab launch takes.
 constexpr int kSlabMaxSteps = 8;
 
-// Relaxed polling (no cache invalidation per iteration); the caller issues one
-// acquire fence after the wait succeeds.
+// Relaxed polling of a neighbour's block counter.
 __device__ bool wait_counter(const unsigned long long* p, unsigned long long target, unsigned int* error) {
     for (int i = 0; i < kSlabSpinLimit; ++i) {
         if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
@@ -337,18 +340,42 @@ __device__ bool wait_counter(const unsigned long long* p, unsigned long long tar
     return false;
 }
 
+// Halo rows cross XCDs, each with its own L2, through device-scope (agent)
+// stores and loads (sc1): such a store is acknowledged once every CU of the
+// device can see it, and such a load reads at that level, so the exchange
+// needs no cache-wide write-back or invalidate. Such stores are written
+// through one by one, so the exchange rows are laid out lane-interleaved: a
+// row is 32*DW dword pairs, and pair k of lane i (dwords 2k, 2k+1 of its
+// segment) sits at slot k*64 + i, so each store instruction of a wavefront
+// writes 512 contiguous bytes. Slots past the grid's width hold zeros.
+template <int DW>
+__device__ __forceinline__ void publish_seg(unsigned long long* row, int lane, const uint32_t (&v)[DW]) {
+    static_assert(DW % 2 == 0, "slab rows are published as dword pairs");
+#pragma unroll
+    for (int k = 0; k < DW / 2; ++k) {
+        const unsigned long long pair =
+            static_cast<unsigned long long>(v[2 * k]) | (static_cast<unsigned long long>(v[2 * k + 1]) << 32);
+        __hip_atomic_store(row + k * 64 + lane, pair, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__device__ __forceinline__ unsigned long long fetch_pair(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // The block loop of one slab; the row and column masks are fixed for the
 // slab, so each instantiation holds one copy of the band update.
 template <int DW, int BR, int NW, int STAGE_ROWS, bool MASK_ROWS, bool MASK_COLS>
 __device__ __forceinline__ bool slab_blocks(const SlabArgs& a, uint32_t (&s)[BR][DW], EdgeRows<DW, NW>& lds,
                                             uint32_t* halo_stage, int* keep_going, int lane, int w, int gx0,
                                             int row_lo, int row_hi, const uint32_t (&colmask)[DW]) {
+    constexpr int RP = 32 * DW;  // dword pairs per exchanged row (one wavefront's 64 segments)
+    constexpr int K = (kSlabMaxSteps * RP + 64 * NW - 1) / (64 * NW);  // pairs per thread and side
     const TileGeo& g = a.geo;
     const int ty = blockIdx.x;
     const int T = a.steps;
     const int tile_rows = NW * BR;
-    const bool vec_ok = (g.x_dw % DW) == 0;
-    const size_t plane = static_cast<size_t>(T) * g.x_dw;  // T rows of one side
+    const size_t plane = static_cast<size_t>(T) * RP;  // pairs of one side: T rows
     const int total_blocks = a.blocks + (a.last_gens > 0 ? 1 : 0);
     for (int b = 0; b < total_blocks; ++b) {
         const int gens = b < a.blocks ? T : a.last_gens;
@@ -356,35 +383,27 @@ __device__ __forceinline__ bool slab_blocks(const SlabArgs& a, uint32_t (&s)[BR]
         if (b + 1 == total_blocks) break;
 
         const int par = b & 1;
-        uint32_t* mine = a.xbuf + (static_cast<size_t>(par) * a.tiles_y + ty) * 2 * plane;
+        unsigned long long* mine = a.xbuf + (static_cast<size_t>(par) * a.tiles_y + ty) * 2 * plane;
 #pragma unroll
         for (int r = 0; r < BR; ++r) {
             const int tr = w * BR + r;
-            if (tr >= T && tr < 2 * T)
-                store_seg<DW>(mine + static_cast<size_t>(tr - T) * g.x_dw, g.x_dw, 0, gx0, vec_ok, s[r]);
+            if (tr >= T && tr < 2 * T) publish_seg<DW>(mine + static_cast<size_t>(tr - T) * RP, lane, s[r]);
             if (tr >= tile_rows - 2 * T && tr < tile_rows - T)
-                store_seg<DW>(mine + plane + static_cast<size_t>(tr - (tile_rows - 2 * T)) * g.x_dw, g.x_dw, 0, gx0,
-                              vec_ok, s[r]);
+                publish_seg<DW>(mine + plane + static_cast<size_t>(tr - (tile_rows - 2 * T)) * RP, lane, s[r]);
             __builtin_amdgcn_sched_barrier(0);
         }
-        // Publish: every wave's stores have left it (vmcnt 0) before the
-        // barrier; then one thread writes back the L2 (release fence), waits
-        // for the write-back to finish (the fence alone leaves no wait between
-        // the write-back and the next store) and only then bumps the counter.
-        // Subscribe: one thread polls both neighbours, then
-        // invalidates the CU's L1 and the L2 once (acquire fence) before any
-        // wave of the workgroup reads their rows.
+        // Publish: every wave waits for its device-scope stores to be
+        // acknowledged (vmcnt 0) before the barrier; then one thread bumps the
+        // counter. Subscribe: the same thread polls both neighbours; after the
+        // barrier every wave reads their rows with device-scope loads.
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         if (threadIdx.x == 0) {
             const unsigned long long target = a.epoch + static_cast<unsigned long long>(b) + 1;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __builtin_amdgcn_s_waitcnt(0);
             __hip_atomic_store(&a.flags[ty], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             bool ok = true;
             if (ty > 0) ok = wait_counter(&a.flags[ty - 1], target, a.error);
             if (ok && ty + 1 < a.tiles_y) ok = wait_counter(&a.flags[ty + 1], target, a.error);
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             *keep_going = ok ? 1 : 0;
         }
         __syncthreads();
@@ -394,27 +413,38 @@ __device__ __forceinline__ bool slab_blocks(const SlabArgs& a, uint32_t (&s)[BR]
         // independent loads (their latencies overlap), then each wavefront
         // takes its halo rows from LDS. Both sides go in one pass when the
         // stage holds 2T rows, else the upper side first, then the lower.
-        const uint32_t* above =
+        const unsigned long long* above =
             a.xbuf + (static_cast<size_t>(par) * a.tiles_y + (ty > 0 ? ty - 1 : 0)) * 2 * plane + plane;
-        const uint32_t* below =
+        const unsigned long long* below =
             a.xbuf + (static_cast<size_t>(par) * a.tiles_y + (ty + 1 < a.tiles_y ? ty + 1 : ty)) * 2 * plane;
         const int passes = 2 * T <= STAGE_ROWS ? 1 : 2;
-        const size_t below_at = passes == 1 ? plane : 0;  // stage offset of the lower side
+        const int below_at = passes == 1 ? T * RP : 0;  // stage pair offset of the lower side
 #pragma nounroll
         for (int pass = 0; pass < passes; ++pass) {
             const bool take_above = (passes == 1 || pass == 0) && ty > 0;
             const bool take_below = (passes == 1 || pass == 1) && ty + 1 < a.tiles_y;
-            if ((plane & 3) == 0) {  // xbuf planes start on 16-byte boundaries
-                const int quads = static_cast<int>(plane >> 2);
-                uint4* stage = reinterpret_cast<uint4*>(halo_stage);
-                for (int i = threadIdx.x; i < quads; i += 64 * NW) {
-                    if (take_above) stage[i] = reinterpret_cast<const uint4*>(above)[i];
-                    if (take_below) stage[(below_at >> 2) + i] = reinterpret_cast<const uint4*>(below)[i];
+            {
+                // every load is issued before the first is used; slots are
+                // clamped to the plane so that no load needs a branch. The
+                // stage holds rows in natural order, 64*DW dwords each.
+                const int pairs = T * RP;
+                unsigned long long va[K], vb[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const int i = min(static_cast<int>(threadIdx.x) + k * 64 * NW, pairs - 1);
+                    va[k] = fetch_pair(above + i);
+                    vb[k] = fetch_pair(below + i);
                 }
-            } else {
-                for (int i = threadIdx.x; i < static_cast<int>(plane); i += 64 * NW) {
-                    if (take_above) halo_stage[i] = above[i];
-                    if (take_below) halo_stage[below_at + i] = below[i];
+                unsigned long long* stage = reinterpret_cast<unsigned long long*>(halo_stage);
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const int i = static_cast<int>(threadIdx.x) + k * 64 * NW;
+                    const int row = i / RP, slot = i % RP;
+                    const int at = row * RP + (slot % 64) * (DW / 2) + slot / 64;  // pair index in natural order
+                    if (i < pairs) {
+                        if (take_above) stage[at] = va[k];
+                        if (take_below) stage[below_at + at] = vb[k];
+                    }
                 }
             }
             __syncthreads();
@@ -424,9 +454,7 @@ __device__ __forceinline__ bool slab_blocks(const SlabArgs& a, uint32_t (&s)[BR]
                 const bool top = take_above && tr < T;
                 const bool bottom = take_below && tr >= tile_rows - T;
                 if (top || bottom) {
-                    const uint32_t* row =
-                        halo_stage + (top ? static_cast<size_t>(tr) * g.x_dw
-                                          : below_at + static_cast<size_t>(tr - (tile_rows - T)) * g.x_dw);
+                    const uint32_t* row = halo_stage + 2 * (top ? tr * RP : below_at + (tr - (tile_rows - T)) * RP);
 #pragma unroll
                     for (int d = 0; d < DW; ++d) s[r][d] = gx0 + d < g.x_dw ? row[gx0 + d] : 0u;
                 }
@@ -450,7 +478,7 @@ template <int DW, int BR, int NW, bool MASK_COLS>
 __global__ __launch_bounds__(64 * NW) void slab_kernel(SlabArgs a) {
     constexpr int STAGE_ROWS = slab_stage_rows<DW, NW>();
     __shared__ EdgeRows<DW, NW> lds;
-    __shared__ uint32_t halo_stage[STAGE_ROWS * 64 * DW];
+    __shared__ __align__(16) uint32_t halo_stage[STAGE_ROWS * 64 * DW];
     __shared__ int keep_going;
     const TileGeo& g = a.geo;
     const int lane = threadIdx.x & 63;
@@ -651,8 +679,8 @@ static const void* slab_function(const TileShape& shape, bool mask_cols) {
     return f;
 }
 
-bool slab_capacity(const TileShape& shape, int* workgroups) {
-    const void* f = slab_function(shape, false);
+bool slab_capacity(const TileShape& shape, bool mask_cols, int* workgroups) {
+    const void* f = slab_function(shape, mask_cols);
     if (!f) return false;
     int dev = 0, cus = 0, coop = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess) return false;

@@ -86,15 +86,16 @@ struct SlabArgs {
     int blocks;                 // full blocks of T generations
     int last_gens;              // generations of a final shorter block (0: none)
     int tiles_y;                // workgroups
-    uint32_t* xbuf;             // [2][tiles_y][2][T][x_dw] exchanged halo rows
+    unsigned long long* xbuf;   // [2][tiles_y][2][T][32 * DW] exchanged halo rows, lane-interleaved pairs
     unsigned long long* flags;  // [tiles_y] blocks published so far, offset by epoch
     unsigned long long epoch;   // flag value every slab holds before this launch
     unsigned int* error;        // raised when a neighbour wait runs out
 };
 
 // Workgroups of the slab kernel that can be resident at once (false if the
-// device cannot launch cooperatively).
-bool slab_capacity(const TileShape& shape, int* workgroups);
+// device cannot launch cooperatively); mask_cols selects the variant for rows
+// narrower than a wavefront's 64 lanes.
+bool slab_capacity(const TileShape& shape, bool mask_cols, int* workgroups);
 hipError_t launch_slab(const TileShape& shape, const SlabArgs& args, hipStream_t stream);
 
 // Plain kernel: one generation per launch, global memory to global memory.
