@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/golhip.h"
 #include "gol_capi_internal.hpp"
@@ -129,11 +130,12 @@ struct gol_handle {
     TileGeometry geom{};
     uint64_t launches = 0;
     uint64_t generations = 0;
-    // change tracking (GOL_KERNEL_ACTIVE)
-    uint32_t* changed = nullptr;  // [2][n_tiles] flags: tile interior changed over a block
-    int* lists = nullptr;         // [2][n_tiles] active-tile lists
-    int* counts = nullptr;        // [2] list lengths
-    unsigned long long* active_total = nullptr;
+    // change tracking (GOL_KERNEL_ACTIVE): small tiles, one wavefront each
+    FineGeo fine{};
+    uint32_t* marks = nullptr;    // [3][n_tiles] tile is on list k (lists rotate by block)
+    int* lists = nullptr;         // [3][n_tiles] active-tile lists
+    int* counts = nullptr;        // [3] list lengths
+    unsigned long long* wg_active = nullptr;  // [fine_workgroups] tiles advanced, per workgroup
     int n_tiles = 0;
     uint64_t blocks = 0;          // change-tracking blocks run
     bool flags_valid = false;     // false: the next block treats every tile as active
@@ -175,9 +177,28 @@ int plan(gol_handle* h) {
         h->plain_dw = dw;
         return GOL_OK;
     }
-    // Change tracking wants narrow tiles (one dword per lane: 2048-cell-wide
-    // tiles) so that quiet regions of a wide grid can be skipped.
-    TileShape s = h->kernel == GOL_KERNEL_ACTIVE ? TileShape{1, 16, 8} : default_shape(h->x_dw, h->y_rows);
+    if (h->kernel == GOL_KERNEL_ACTIVE) {
+        // Change tracking skips quiet regions tile by tile, so its tiles are
+        // small: 256 cells x (64 - 2T) rows, one wavefront each. 12
+        // generations per block by default: oscillators of period 1, 2, 3, 4,
+        // 6 and 12 return to their state at block boundaries and count as
+        // unchanged. The tile shape fields of gol_config do not apply.
+        const int steps = c.temporal_steps ? static_cast<int>(c.temporal_steps) : 12;
+        if (steps < 1 || steps > kFineMaxSteps)
+            return fail(GOL_ERR_INVALID_ARGUMENT, "temporal_steps %d: change tracking takes 1 to %d generations per block",
+                        steps, kFineMaxSteps);
+        h->steps = steps;
+        h->fine.x_dw = static_cast<int>(h->x_dw);
+        h->fine.y_rows = static_cast<int>(h->y_rows);
+        h->fine.halo_rows = steps;
+        h->fine.tile_rows = 64 - 2 * steps;
+        h->fine.tiles_x = static_cast<int>((h->x_dw + kFineDW - 1) / kFineDW);
+        h->fine.tiles_y = static_cast<int>((h->y_rows + h->fine.tile_rows - 1) / h->fine.tile_rows);
+        h->n_tiles = h->fine.tiles_x * h->fine.tiles_y;
+        h->shape = TileShape{kFineDW, 1, kFineWaves};
+        return GOL_OK;
+    }
+    TileShape s = default_shape(h->x_dw, h->y_rows);
     int modelled_steps = -1;
     if (h->kernel == GOL_KERNEL_TEMPORAL) {
         TileShape m;
@@ -197,11 +218,8 @@ int plan(gol_handle* h) {
     h->shape = s;
     h->one_tile = h->x_dw <= 64 * s.lane_dwords && h->y_rows <= s.band_rows * s.waves;
     int steps = static_cast<int>(c.temporal_steps);
-    // 12 generations per block: oscillators of period 1, 2, 3, 4, 6 and 12
-    // return to their state at block boundaries and count as unchanged.
     if (steps == 0 && modelled_steps > 0 && !h->one_tile) steps = modelled_steps;
-    if (steps == 0) steps = h->kernel == GOL_KERNEL_ACTIVE ? 12 : (h->one_tile ? 1024 : 8);
-    if (h->kernel == GOL_KERNEL_ACTIVE) h->one_tile = false;  // blocks keep their length
+    if (steps == 0) steps = h->one_tile ? 1024 : 8;
     if (steps < 1 || steps > 1 << 20) return fail(GOL_ERR_INVALID_ARGUMENT, "temporal_steps out of range");
     const char* why = nullptr;
     if (!plan_tiles(s, h->x_dw, h->y_rows, h->one_tile ? 1 : steps, &h->geom, &why))
@@ -270,34 +288,39 @@ int enqueue_persistent(gol_handle* h, uint64_t generations) {
     return GOL_OK;
 }
 
-// One change-tracking block of h->steps generations:
-//   build the active list from the last block's flags (every tile when the
-//   flags are not valid), then advance only the listed tiles. A skipped tile
-//   keeps the state it had two blocks ago in the output buffer, which equals
-//   its current state: it and its neighbours did not change over the last
-//   block, so the block starting now reproduces the same state (the tiles are
-//   at least `steps` cells wide and tall, the reach of `steps` generations).
+// One change-tracking block of h->steps generations, one launch: the tiles
+// on this block's list (every tile when the lists are not valid) advance, and
+// each tile that changed puts its 3 x 3 neighbourhood on the next block's
+// list. A skipped tile keeps the state it had two blocks ago in the output
+// buffer, which equals its current state: it and its neighbours did not
+// change over the last block, so the block starting now reproduces the same
+// state (the tiles are at least `steps` cells wide and tall, the reach of
+// `steps` generations). Lists, marks and counters rotate over three slots:
+// block b reads slot b % 3, appends to slot (b + 1) % 3 and resets the
+// counter of slot (b + 2) % 3, which block b - 1 read.
 int enqueue_active_block(gol_handle* h) {
-    const int k = static_cast<int>(h->blocks & 1);
+    const size_t n = static_cast<size_t>(h->n_tiles);
     hipError_t e;
     if (!h->flags_valid) {
-        // the previous-block flags say "changed" everywhere
-        if ((e = hipMemsetAsync(h->changed + (k ^ 1) * h->n_tiles, 0xff, h->n_tiles * sizeof(uint32_t), h->stream)) !=
-            hipSuccess)
+        // a new state: every tile advances, and no list or mark is pending
+        if ((e = hipMemsetAsync(h->marks, 0, 3 * n * sizeof(uint32_t), h->stream)) != hipSuccess ||
+            (e = hipMemsetAsync(h->counts, 0, 3 * sizeof(int), h->stream)) != hipSuccess)
             return fail_hip(e, "hipMemsetAsync");
-        if ((e = hipMemsetAsync(h->counts + k, 0, sizeof(int), h->stream)) != hipSuccess)
-            return fail_hip(e, "hipMemsetAsync");
-        h->flags_valid = true;
     }
-    e = launch_build_active_list(h->changed + (k ^ 1) * h->n_tiles, h->changed + k * h->n_tiles, h->geom.tiles_x,
-                                 h->geom.tiles_y, h->lists + k * h->n_tiles, h->counts + k, h->counts + (k ^ 1),
-                                 h->active_total, h->stream);
-    if (e != hipSuccess) return fail_hip(e, "build_active_list launch");
-    e = launch_active_tiles(h->geom, h->buf[h->cur], h->buf[h->cur ^ 1], h->x_dw, h->y_rows, h->steps,
-                            h->lists + k * h->n_tiles, h->counts + k, h->changed + k * h->n_tiles, h->stream);
-    if (e != hipSuccess) return fail_hip(e, "active_tile_kernel launch");
+    const int k = static_cast<int>(h->blocks % 3), k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+    FineTrack tr;
+    tr.mark_cur = h->marks + k * n;
+    tr.mark_next = h->marks + k1 * n;
+    tr.list_next = h->lists + k1 * n;
+    tr.count_next = h->counts + k1;
+    tr.count_clear = h->counts + k2;
+    tr.wg_active = h->wg_active;
+    e = launch_fine_tiles(h->fine, h->buf[h->cur], h->buf[h->cur ^ 1], h->steps,
+                          h->flags_valid ? h->lists + k * n : nullptr, h->counts + k, tr, h->stream);
+    if (e != hipSuccess) return fail_hip(e, "fine_tile_kernel launch");
+    h->flags_valid = true;
     h->cur ^= 1;
-    h->launches += 2;
+    h->launches += 1;
     h->blocks += 1;
     h->generations += static_cast<uint64_t>(h->steps);
     return GOL_OK;
@@ -324,6 +347,9 @@ int enqueue(gol_handle* h, uint64_t generations) {
         if (h->kernel == GOL_KERNEL_PLAIN) {
             e = launch_plain(static_cast<int>(h->plain_dw), src, dst, h->x_dw, h->y_rows, h->stream);
             done = 1;
+        } else if (h->kernel == GOL_KERNEL_ACTIVE) {
+            done = std::min<uint64_t>(left, static_cast<uint64_t>(h->steps));
+            e = launch_fine_tiles(h->fine, src, dst, static_cast<int>(done), nullptr, nullptr, FineTrack{}, h->stream);
         } else {
             done = std::min<uint64_t>(left, static_cast<uint64_t>(h->steps));
             e = launch_tiles(h->geom, src, dst, h->x_dw, h->y_rows, static_cast<int>(done), h->stream);
@@ -392,12 +418,12 @@ int gol_create(const gol_config* config, gol_handle** out) {
     }
     if (h->kernel == GOL_KERNEL_ACTIVE) {
         const size_t nt = static_cast<size_t>(h->n_tiles);
-        if ((e = hipMalloc(&h->changed, 2 * nt * sizeof(uint32_t))) != hipSuccess ||
-            (e = hipMalloc(&h->lists, 2 * nt * sizeof(int))) != hipSuccess ||
-            (e = hipMalloc(&h->counts, 2 * sizeof(int))) != hipSuccess ||
-            (e = hipMalloc(&h->active_total, sizeof(unsigned long long))) != hipSuccess ||
-            (e = hipMemsetAsync(h->counts, 0, 2 * sizeof(int), h->stream)) != hipSuccess ||
-            (e = hipMemsetAsync(h->active_total, 0, sizeof(unsigned long long), h->stream)) != hipSuccess) {
+        const size_t nwg = static_cast<size_t>(fine_workgroups(h->fine));
+        if ((e = hipMalloc(&h->marks, 3 * nt * sizeof(uint32_t))) != hipSuccess ||
+            (e = hipMalloc(&h->lists, 3 * nt * sizeof(int))) != hipSuccess ||
+            (e = hipMalloc(&h->counts, 3 * sizeof(int))) != hipSuccess ||
+            (e = hipMalloc(&h->wg_active, nwg * sizeof(unsigned long long))) != hipSuccess ||
+            (e = hipMemsetAsync(h->wg_active, 0, nwg * sizeof(unsigned long long), h->stream)) != hipSuccess) {
             rc = fail_hip(e, "gol_create (change tracking state)");
             gol_destroy(h);
             return rc;
@@ -525,7 +551,8 @@ int gol_get_info(gol_handle* h, gol_info* info) {
         info->band_rows = static_cast<uint32_t>(h->shape.band_rows);
         info->waves = static_cast<uint32_t>(h->shape.waves);
         info->temporal_steps = static_cast<uint32_t>(h->steps);
-        info->tiles = static_cast<uint32_t>(h->geom.tiles_x * h->geom.tiles_y);
+        info->tiles = static_cast<uint32_t>(h->kernel == GOL_KERNEL_ACTIVE ? h->n_tiles
+                                                                           : h->geom.tiles_x * h->geom.tiles_y);
         info->persistent = h->persistent ? 1u : 0u;
     }
     info->launches = h->launches;
@@ -533,13 +560,17 @@ int gol_get_info(gol_handle* h, gol_info* info) {
     info->device_bytes = 2 * h->n_dwords * 4;
     if (h->kernel == GOL_KERNEL_ACTIVE) {
         DeviceGuard g(h->device);
-        unsigned long long total = 0;
-        hipError_t e = hipMemcpyAsync(&total, h->active_total, sizeof(total), hipMemcpyDeviceToHost, h->stream);
+        std::vector<unsigned long long> per_wg(static_cast<size_t>(fine_workgroups(h->fine)));
+        hipError_t e = hipMemcpyAsync(per_wg.data(), h->wg_active, per_wg.size() * sizeof(unsigned long long),
+                                      hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
         if (e != hipSuccess) return fail_hip(e, "gol_get_info");
+        unsigned long long total = 0;
+        for (unsigned long long v : per_wg) total += v;
         info->active_tiles_total = total;
         info->tracked_blocks = h->blocks;
-        info->device_bytes += static_cast<uint64_t>(h->n_tiles) * 2 * (sizeof(uint32_t) + sizeof(int)) + 16;
+        info->device_bytes += static_cast<uint64_t>(h->n_tiles) * 3 * (sizeof(uint32_t) + sizeof(int)) +
+                              3 * sizeof(int) + per_wg.size() * sizeof(unsigned long long);
     }
     return GOL_OK;
 }
@@ -551,10 +582,10 @@ void gol_destroy(gol_handle* h) {
         if (h->stream) (void)hipStreamSynchronize(h->stream);
         if (h->buf[0]) (void)hipFree(h->buf[0]);
         if (h->buf[1]) (void)hipFree(h->buf[1]);
-        if (h->changed) (void)hipFree(h->changed);
+        if (h->marks) (void)hipFree(h->marks);
         if (h->lists) (void)hipFree(h->lists);
         if (h->counts) (void)hipFree(h->counts);
-        if (h->active_total) (void)hipFree(h->active_total);
+        if (h->wg_active) (void)hipFree(h->wg_active);
         if (h->xbuf) (void)hipFree(h->xbuf);
         if (h->slab_flags) (void)hipFree(h->slab_flags);
         if (h->slab_error) (void)hipFree(h->slab_error);

@@ -219,10 +219,9 @@ __device__ __forceinline__ void run_generations(uint32_t (&s)[BR][DW], EdgeRows<
     }
 }
 
-// Returns (in every lane) whether any stored interior dword differs from src
-// when TRACK is set.
-template <int DW, int BR, int NW, bool TRACK>
-__device__ __forceinline__ bool process_tile(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+// Advances one tile by gens generations and stores its interior.
+template <int DW, int BR, int NW>
+__device__ __forceinline__ void process_tile(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
                                              const TileGeo& a, int tile, int gens, EdgeRows<DW, NW>& lds) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
@@ -258,44 +257,157 @@ __device__ __forceinline__ bool process_tile(const uint32_t* __restrict__ src, u
 
     const int tile_rows = NW * BR;
     const bool lane_inside = lane >= a.halo_lanes && lane < 64 - a.halo_lanes;
-    bool changed = false;
 #pragma unroll
     for (int r = 0; r < BR; ++r) {
         const int tr = w * BR + r;
         const int gy = gy0 + r;
-        if (tr >= a.halo_rows && tr < tile_rows - a.halo_rows && gy >= 0 && gy < a.y_rows && lane_inside) {
-            if (TRACK) {
-                uint32_t before[DW];
-                load_seg<DW>(src, a.x_dw, a.y_rows, gy, gx0, vec_ok, before);
-#pragma unroll
-                for (int d = 0; d < DW; ++d) changed |= (before[d] != s[r][d]);
-            }
+        if (tr >= a.halo_rows && tr < tile_rows - a.halo_rows && gy >= 0 && gy < a.y_rows && lane_inside)
             store_seg<DW>(dst, a.x_dw, gy, gx0, vec_ok, s[r]);
-        }
     }
-    return changed;
 }
 
 template <int DW, int BR, int NW>
 __global__ __launch_bounds__(64 * NW) void tile_kernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
                                                        TileGeo a, int gens) {
     __shared__ EdgeRows<DW, NW> lds;
-    process_tile<DW, BR, NW, false>(src, dst, a, blockIdx.x, gens, lds);
+    process_tile<DW, BR, NW>(src, dst, a, blockIdx.x, gens, lds);
 }
 
-// Change tracking: workgroup i advances tile list[i] (i < *count) and raises
-// changed[tile] when its interior differs after the block of generations.
-template <int DW, int BR, int NW>
-__global__ __launch_bounds__(64 * NW) void active_tile_kernel(const uint32_t* __restrict__ src,
-                                                              uint32_t* __restrict__ dst, TileGeo a, int gens,
-                                                              const int* __restrict__ list,
-                                                              const int* __restrict__ count,
-                                                              uint32_t* __restrict__ changed) {
-    __shared__ EdgeRows<DW, NW> lds;
-    if (static_cast<int>(blockIdx.x) >= *count) return;  // whole workgroup leaves before any barrier
-    const int tile = list[blockIdx.x];
-    const bool c = process_tile<DW, BR, NW, true>(src, dst, a, tile, gens, lds);
-    if (__ballot(c) != 0 && (threadIdx.x & 63) == 0) changed[tile] = 1u;
+// ---------------------------------------------------------------------------
+// Change tracking over small tiles. A wavefront advances one tile of kFineDW
+// dwords (256 cells) x tile_rows rows by up to T generations: lane l holds
+// grid row y0 - T + l, so the wavefront spans the tile's rows and T halo rows
+// above and below (tile_rows = 64 - 2T), as kFineDW + 2 dwords: the tile's
+// dwords plus one halo dword on each side (enough for T <= 32). Rows above and
+// below a lane's row come from the neighbouring lanes (DPP), a dword's west
+// and east neighbours from the lane's own dwords, so a wavefront needs no LDS
+// and no barrier, and the wavefronts of a workgroup take independent tiles.
+// A halo cell is wrong after g generations only if it lies within g cells of
+// the wavefront's edge; after T generations the tile's own cells are exact.
+// ---------------------------------------------------------------------------
+constexpr int kFineN = kFineDW + 2;  // dwords held per lane
+
+template <bool MASK>
+__device__ __forceinline__ void fine_generations(uint32_t (&s)[kFineN], int gens, uint32_t rowmask,
+                                                 const uint32_t (&colmask)[kFineN]) {
+    for (int g = 0; g < gens; ++g) {
+        uint32_t h0[kFineN], h1[kFineN], e0[kFineN], e1[kFineN];
+#pragma unroll
+        for (int j = 0; j < kFineN; ++j) {
+            const uint32_t w = west_of(j > 0 ? s[j - 1] : 0u, s[j]);
+            const uint32_t e = east_of(s[j], j + 1 < kFineN ? s[j + 1] : 0u);
+            const RowSum r = row_sum(w, s[j], e);
+            h0[j] = r.h0;
+            h1[j] = r.h1;
+            e0[j] = r.e0;
+            e1[j] = r.e1;
+        }
+#pragma unroll
+        for (int j = 0; j < kFineN; ++j) {
+            const uint32_t a0 = from_lower_lane(h0[j]), a1 = from_lower_lane(h1[j]);  // row above
+            const uint32_t b0 = from_upper_lane(h0[j]), b1 = from_upper_lane(h1[j]);  // row below
+            uint32_t v = next_state(a0, a1, e0[j], e1[j], b0, b1, s[j]);
+            if (MASK) v &= rowmask & colmask[j];
+            s[j] = v;
+        }
+    }
+}
+
+// Wavefront i of the launch advances tile list[i] (i < *count), or tile i
+// (i < dense_tiles) when list is null, by gens generations. With tracking
+// (tr.list_next set), a tile whose cells changed appends the tiles of its
+// 3 x 3 neighbourhood to the next block's list: a tile joins once, when its
+// mark goes from 0 to 1, and a wavefront reserves its entries with one atomic
+// add. A wavefront clears the mark of the listed tile it advances (its list
+// is done with it), and the launch resets the counter of the list after next.
+__global__ __launch_bounds__(64 * kFineWaves) void fine_tile_kernel(const uint32_t* __restrict__ src,
+                                                                    uint32_t* __restrict__ dst, FineGeo g, int gens,
+                                                                    const int* __restrict__ list,
+                                                                    const int* __restrict__ count, int dense_tiles,
+                                                                    FineTrack tr) {
+    if (tr.count_clear && blockIdx.x == 0 && threadIdx.x == 0) *tr.count_clear = 0;
+    const int slot = blockIdx.x * kFineWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (slot >= (list ? *count : dense_tiles)) return;  // whole wavefronts leave; there are no barriers
+    const int tile = list ? __builtin_amdgcn_readfirstlane(list[slot]) : slot;
+    const int lane = threadIdx.x & 63;
+    const int tx = tile % g.tiles_x, ty = tile / g.tiles_x;
+    const int x0 = tx * kFineDW - 1;                 // first dword held: the west halo dword
+    const int y0 = ty * g.tile_rows - g.halo_rows;   // row of lane 0
+    const int y = y0 + lane;
+    const bool row_in = y >= 0 && y < g.y_rows;
+    const uint32_t* row = src + static_cast<size_t>(row_in ? y : 0) * g.x_dw;
+
+    uint32_t s[kFineN], colmask[kFineN];
+#pragma unroll
+    for (int j = 0; j < kFineN; ++j) {
+        const int gx = x0 + j;
+        colmask[j] = (gx >= 0 && gx < g.x_dw) ? ~0u : 0u;
+    }
+    // the tile's own dwords: two 16-byte loads when all eight lie in the row
+    // and rows start on 16-byte boundaries (x0 + 1 is a multiple of 8 dwords),
+    // else one clamped load per dword
+    const bool full_x = x0 + 1 + kFineDW <= g.x_dw && (g.x_dw & 3) == 0;
+    if (full_x) {
+        const uint4* v = reinterpret_cast<const uint4*>(row + x0 + 1);
+        const uint4 lo = v[0], hi = v[1];
+        s[1] = lo.x; s[2] = lo.y; s[3] = lo.z; s[4] = lo.w;
+        s[5] = hi.x; s[6] = hi.y; s[7] = hi.z; s[8] = hi.w;
+    } else {
+#pragma unroll
+        for (int j = 1; j <= kFineDW; ++j) s[j] = row[colmask[j] ? x0 + j : 0] & colmask[j];
+    }
+    s[0] = row[colmask[0] ? x0 : 0] & colmask[0];
+    s[kFineN - 1] = row[colmask[kFineN - 1] ? x0 + kFineN - 1 : 0] & colmask[kFineN - 1];
+    const uint32_t rowmask = row_in ? ~0u : 0u;
+#pragma unroll
+    for (int j = 0; j < kFineN; ++j) s[j] &= rowmask;
+
+    uint32_t before[kFineDW];
+#pragma unroll
+    for (int d = 0; d < kFineDW; ++d) before[d] = s[1 + d];
+
+    const bool edge = y0 < 0 || y0 + 64 > g.y_rows || x0 < 0 || x0 + kFineN > g.x_dw;  // wave-uniform
+    if (edge) fine_generations<true>(s, gens, rowmask, colmask);
+    else fine_generations<false>(s, gens, rowmask, colmask);
+
+    const bool own = lane >= g.halo_rows && lane < g.halo_rows + g.tile_rows && row_in;
+    bool diff = false;
+#pragma unroll
+    for (int d = 0; d < kFineDW; ++d) diff |= before[d] != s[1 + d];
+    if (own) {
+        uint32_t* out = dst + static_cast<size_t>(y) * g.x_dw;
+        if (full_x) {
+            uint4* v = reinterpret_cast<uint4*>(out + x0 + 1);
+            v[0] = make_uint4(s[1], s[2], s[3], s[4]);
+            v[1] = make_uint4(s[5], s[6], s[7], s[8]);
+        } else {
+#pragma unroll
+            for (int j = 1; j <= kFineDW; ++j)
+                if (colmask[j]) out[x0 + j] = s[j];
+        }
+    }
+    if (!tr.list_next) return;
+    if (lane == 0) {
+        if (list) tr.mark_cur[tile] = 0u;
+        atomicAdd(&tr.wg_active[blockIdx.x], 1ull);
+    }
+    if (__ballot(own && diff) == 0) return;
+    // lanes 0..8: the 3 x 3 neighbourhood joins the next block's list
+    int n = 0;
+    bool add = false;
+    if (lane < 9) {
+        const int nx = tx + lane % 3 - 1, ny = ty + lane / 3 - 1;
+        if (nx >= 0 && nx < g.tiles_x && ny >= 0 && ny < g.tiles_y) {
+            n = ny * g.tiles_x + nx;
+            add = atomicOr(&tr.mark_next[n], 1u) == 0u;
+        }
+    }
+    const unsigned long long m = __ballot(add);
+    if (m == 0) return;
+    int base = 0;
+    if (lane == 0) base = atomicAdd(tr.count_next, __popcll(m));
+    base = __shfl(base, 0);
+    if (add) tr.list_next[base + __popcll(m & ((1ull << lane) - 1))] = n;
 }
 
 // Batched grids: workgroup i advances work item i (a tile of one grid).
@@ -307,7 +419,7 @@ __global__ __launch_bounds__(64 * NW) void item_kernel(const BatchGrid* __restri
     const BatchGrid& g = grids[it.grid];
     const uint32_t* src = g.buf[it.parity];
     uint32_t* dst = g.buf[it.parity ^ 1];
-    process_tile<DW, BR, NW, false>(src, dst, g.geo, it.tile, it.gens, lds);
+    process_tile<DW, BR, NW>(src, dst, g.geo, it.tile, it.gens, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -515,31 +627,6 @@ __global__ __launch_bounds__(64 * NW) void slab_kernel(SlabArgs a) {
     }
 }
 
-// Active-tile list for the next block: tile t is active when it or one of its
-// eight neighbours changed in the last block. Also clears the flags the next
-// block writes and the counter of the list after next.
-__global__ void build_active_list(const uint32_t* __restrict__ changed_prev, uint32_t* __restrict__ changed_next,
-                                  int tiles_x, int tiles_y, int* __restrict__ list, int* __restrict__ count,
-                                  int* __restrict__ count_other, unsigned long long* __restrict__ total) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0) *count_other = 0;
-    if (t >= tiles_x * tiles_y) return;
-    const int tx = t % tiles_x, ty = t / tiles_x;
-    bool act = false;
-    for (int dy = -1; dy <= 1; ++dy) {
-        for (int dx = -1; dx <= 1; ++dx) {
-            const int nx = tx + dx, ny = ty + dy;
-            if (nx >= 0 && nx < tiles_x && ny >= 0 && ny < tiles_y) act |= changed_prev[ny * tiles_x + nx] != 0;
-        }
-    }
-    changed_next[t] = 0;
-    if (act) {
-        const int i = atomicAdd(count, 1);
-        list[i] = t;
-        atomicAdd(total, 1ull);
-    }
-}
-
 }  // namespace
 
 const TileShape kTileShapes[] = {
@@ -623,28 +710,12 @@ hipError_t launch_tiles(const TileGeometry& g, const uint32_t* src, uint32_t* ds
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_active_tiles(const TileGeometry& g, const uint32_t* src, uint32_t* dst, int64_t x_dw,
-                               int64_t y_rows, int gens, const int* list, const int* count, uint32_t* changed,
-                               hipStream_t stream) {
-    const unsigned blocks = static_cast<unsigned>(g.tiles_x) * static_cast<unsigned>(g.tiles_y);
-#define GOLHIP_LAUNCH(DW, BR, NW)                                                                        \
-    if (g.lane_dwords == DW && g.band_rows == BR && g.waves == NW) {                                     \
-        const TileGeo a = to_device_geo(g, x_dw, y_rows);                                                \
-        hipLaunchKernelGGL((active_tile_kernel<DW, BR, NW>), dim3(blocks), dim3(64 * NW), 0, stream, src, dst, \
-                           a, gens, list, count, changed);                                               \
-        return hipGetLastError();                                                                        \
-    }
-    GOLHIP_FOR_EACH_SHAPE(GOLHIP_LAUNCH)
-#undef GOLHIP_LAUNCH
-    return hipErrorInvalidValue;
-}
-
-hipError_t launch_build_active_list(const uint32_t* changed_prev, uint32_t* changed_next, int tiles_x, int tiles_y,
-                                    int* list, int* count, int* count_other, unsigned long long* total,
-                                    hipStream_t stream) {
-    const int n = tiles_x * tiles_y;
-    hipLaunchKernelGGL(build_active_list, dim3((n + 255) / 256), dim3(256), 0, stream, changed_prev, changed_next,
-                       tiles_x, tiles_y, list, count, count_other, total);
+hipError_t launch_fine_tiles(const FineGeo& g, const uint32_t* src, uint32_t* dst, int gens, const int* list,
+                             const int* count, const FineTrack& track, hipStream_t stream) {
+    const int n = g.tiles_x * g.tiles_y;
+    const unsigned blocks = static_cast<unsigned>(fine_workgroups(g));
+    hipLaunchKernelGGL(fine_tile_kernel, dim3(blocks), dim3(64 * kFineWaves), 0, stream, src, dst, g, gens, list, count,
+                       n, track);
     return hipGetLastError();
 }
 

@@ -61,17 +61,38 @@ hipError_t launch_tiles(const TileGeometry& g, const uint32_t* src, uint32_t* ds
 
 TileGeo to_device_geo(const TileGeometry& g, int64_t x_dw, int64_t y_rows);
 
-// Change tracking: advance the tiles in list[0 .. *count) and raise
-// changed[tile] for tiles whose interior changed over the block.
-hipError_t launch_active_tiles(const TileGeometry& g, const uint32_t* src, uint32_t* dst, int64_t x_dw,
-                               int64_t y_rows, int gens, const int* list, const int* count, uint32_t* changed,
-                               hipStream_t stream);
+// Change tracking over small tiles: one wavefront per tile of kFineDW dwords
+// (256 cells) x tile_rows rows, lane l holding row y0 + l of the tile's rows
+// and halo_rows (= T) halo rows above and below: tile_rows = 64 - 2T.
+constexpr int kFineDW = 8;      // dwords of a tile's row
+constexpr int kFineWaves = 4;   // wavefronts (tiles) per workgroup
+constexpr int kFineMaxSteps = 21;  // T <= tile_rows: a tile is at least as tall as T generations reach
 
-// Build the next active list from the last block's changed flags (tile or
-// any of its 8 neighbours changed); clears changed_next and *count_other.
-hipError_t launch_build_active_list(const uint32_t* changed_prev, uint32_t* changed_next, int tiles_x, int tiles_y,
-                                    int* list, int* count, int* count_other, unsigned long long* total,
-                                    hipStream_t stream);
+struct FineGeo {
+    int x_dw, y_rows;
+    int tiles_x, tiles_y;
+    int tile_rows;  // 64 - 2 * halo_rows
+    int halo_rows;  // T: generations per block
+};
+
+// Workgroups of a change-tracking launch (kFineWaves tiles each).
+inline int fine_workgroups(const FineGeo& g) { return (g.tiles_x * g.tiles_y + kFineWaves - 1) / kFineWaves; }
+
+// Change-tracking state of one block (all null: no tracking).
+struct FineTrack {
+    uint32_t* mark_cur;             // marks of the list being advanced: cleared as its tiles are taken
+    uint32_t* mark_next;            // marks of the next block's list (all 0 when the block starts)
+    int* list_next;                 // next block's list, built by appending
+    int* count_next;                // its length (0 when the block starts)
+    int* count_clear;               // length of the list after next: reset to 0
+    unsigned long long* wg_active;  // tiles advanced, per workgroup
+};
+
+// One change-tracking block: advance tiles list[0 .. *count), or every tile
+// when list is null, by gens (<= halo_rows) generations; with track set, the
+// tiles whose cells changed append their 3 x 3 neighbourhoods to the next list.
+hipError_t launch_fine_tiles(const FineGeo& g, const uint32_t* src, uint32_t* dst, int gens, const int* list,
+                             const int* count, const FineTrack& track, hipStream_t stream);
 
 // Batched grids: one workgroup per work item, all items of one tile shape.
 hipError_t launch_items(const TileShape& shape, const BatchGrid* grids, const WorkItem* items, int n_items,

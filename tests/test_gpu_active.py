@@ -40,6 +40,27 @@ def test_patches_against_naive(steps):
         assert info.tracked_blocks == gens // steps
 
 
+# Tiles are 256 cells x (64 - 2T) rows: partial last tile columns and rows,
+# odd dword counts (rows-32), grids shorter than a wavefront, T = 1 and 21.
+@pytest.mark.parametrize("x,y,steps,word_bits", [
+    (32, 1, 12, 32), (96, 5, 3, 32), (2112, 70, 12, 64), (288, 131, 21, 32), (4128, 300, 1, 32),
+    (640, 64, 20, 64), (256, 22, 21, 64), (1344, 257, 7, 64),
+])
+def test_ragged_sizes_against_naive(x, y, steps, word_bits):
+    c = ol.bernoulli_cells(x, y, x * 7 + y, 0.4)
+    gens = 3 * steps + 2
+    note("active ragged", x, y, "steps", steps, "word_bits", word_bits, "gens", gens)
+    out, info = run(ol.pack(c, word_bits), x, y, gens, word_bits=word_bits, temporal_steps=steps)
+    assert np.array_equal(ol.unpack(out, x, y, word_bits), ol.naive_run(c, gens))
+    assert info.tiles == -(-x // 256) * -(-y // (64 - 2 * steps))
+    assert info.tracked_blocks == gens // steps
+
+
+def test_block_length_limit():
+    with pytest.raises(golhip.GolError):
+        golhip.Grid(2048, 2048, kernel=golhip.KERNEL_ACTIVE, temporal_steps=22)
+
+
 def test_skips_tiles_on_a_stable_grid():
     x = y = 8192
     w = golhip.fill_patches(x, y, 5, 4, 64, 0.35)
