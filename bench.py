@@ -99,6 +99,7 @@ def leg_mostly_stable() -> dict:
         "speedup_vs_dense": dense_ms / active_ms,
         "active_tile_fraction": info.active_tiles_total / (info.tiles * info.tracked_blocks),
         "tiles": info.tiles,
+        "tile_cells": f"{32 * info.lane_dwords}x{64 - 2 * info.temporal_steps}",
         "generations_per_block": info.temporal_steps,
         "bit_exact_vs_oracle": exact,
         "blocks_counted": blocks,

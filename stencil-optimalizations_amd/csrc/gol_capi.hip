@@ -179,23 +179,33 @@ int plan(gol_handle* h) {
     }
     if (h->kernel == GOL_KERNEL_ACTIVE) {
         // Change tracking skips quiet regions tile by tile, so its tiles are
-        // small: 256 cells x (64 - 2T) rows, one wavefront each. 12
-        // generations per block by default: oscillators of period 1, 2, 3, 4,
-        // 6 and 12 return to their state at block boundaries and count as
-        // unchanged. The tile shape fields of gol_config do not apply.
+        // small: 32 * lane_dwords cells (lane_dwords 4 or 8) x (64 - 2T)
+        // rows, one wavefront each. 12 generations per block by default:
+        // oscillators of period 1, 2, 3, 4, 6 and 12 return to their state at
+        // block boundaries and count as unchanged. band_rows and waves do not
+        // apply.
         const int steps = c.temporal_steps ? static_cast<int>(c.temporal_steps) : 12;
         if (steps < 1 || steps > kFineMaxSteps)
             return fail(GOL_ERR_INVALID_ARGUMENT, "temporal_steps %d: change tracking takes 1 to %d generations per block",
                         steps, kFineMaxSteps);
+        const int dw = c.lane_dwords ? static_cast<int>(c.lane_dwords) : kFineDefaultDW;
+        if (dw != 4 && dw != 8)
+            return fail(GOL_ERR_INVALID_ARGUMENT, "lane_dwords %d: change-tracking tiles are 4 or 8 dwords wide", dw);
         h->steps = steps;
         h->fine.x_dw = static_cast<int>(h->x_dw);
         h->fine.y_rows = static_cast<int>(h->y_rows);
+        h->fine.tile_dwords = dw;
         h->fine.halo_rows = steps;
         h->fine.tile_rows = 64 - 2 * steps;
-        h->fine.tiles_x = static_cast<int>((h->x_dw + kFineDW - 1) / kFineDW);
+        h->fine.tiles_x = static_cast<int>((h->x_dw + dw - 1) / dw);
         h->fine.tiles_y = static_cast<int>((h->y_rows + h->fine.tile_rows - 1) / h->fine.tile_rows);
         h->n_tiles = h->fine.tiles_x * h->fine.tiles_y;
-        h->shape = TileShape{kFineDW, 1, kFineWaves};
+        // wavefronts loop over list slots: launch no more than the device
+        // holds at once
+        const int all = (h->n_tiles + kFineWaves - 1) / kFineWaves;
+        const int resident = fine_resident_workgroups(dw);
+        h->fine.workgroups = resident > 0 ? std::min(all, resident) : all;
+        h->shape = TileShape{dw, 1, kFineWaves};
         return GOL_OK;
     }
     TileShape s = default_shape(h->x_dw, h->y_rows);

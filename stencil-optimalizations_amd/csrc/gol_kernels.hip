@@ -274,28 +274,27 @@ __global__ __launch_bounds__(64 * NW) void tile_kernel(const uint32_t* __restric
 }
 
 // ---------------------------------------------------------------------------
-// Change tracking over small tiles. A wavefront advances one tile of kFineDW
-// dwords (256 cells) x tile_rows rows by up to T generations: lane l holds
+// Change tracking over small tiles. A wavefront advances one tile of DW
+// dwords (32*DW cells) x tile_rows rows by up to T generations: lane l holds
 // grid row y0 - T + l, so the wavefront spans the tile's rows and T halo rows
-// above and below (tile_rows = 64 - 2T), as kFineDW + 2 dwords: the tile's
-// dwords plus one halo dword on each side (enough for T <= 32). Rows above and
-// below a lane's row come from the neighbouring lanes (DPP), a dword's west
-// and east neighbours from the lane's own dwords, so a wavefront needs no LDS
-// and no barrier, and the wavefronts of a workgroup take independent tiles.
-// A halo cell is wrong after g generations only if it lies within g cells of
-// the wavefront's edge; after T generations the tile's own cells are exact.
+// above and below (tile_rows = 64 - 2T), as DW + 2 dwords: the tile's dwords
+// plus one halo dword on each side (enough for T <= 32). Rows above and below
+// a lane's row come from the neighbouring lanes (DPP), a dword's west and east
+// neighbours from the lane's own dwords, so a wavefront needs no LDS and no
+// barrier, and the wavefronts of a workgroup take independent tiles. A halo
+// cell is wrong after g generations only if it lies within g cells of the
+// wavefront's edge; after T generations the tile's own cells are exact.
 // ---------------------------------------------------------------------------
-constexpr int kFineN = kFineDW + 2;  // dwords held per lane
-
-template <bool MASK>
-__device__ __forceinline__ void fine_generations(uint32_t (&s)[kFineN], int gens, uint32_t rowmask,
-                                                 const uint32_t (&colmask)[kFineN]) {
+template <int DW, bool MASK>
+__device__ __forceinline__ void fine_generations(uint32_t (&s)[DW + 2], int gens, uint32_t rowmask,
+                                                 const uint32_t (&colmask)[DW + 2]) {
+    constexpr int N = DW + 2;
     for (int g = 0; g < gens; ++g) {
-        uint32_t h0[kFineN], h1[kFineN], e0[kFineN], e1[kFineN];
+        uint32_t h0[N], h1[N], e0[N], e1[N];
 #pragma unroll
-        for (int j = 0; j < kFineN; ++j) {
+        for (int j = 0; j < N; ++j) {
             const uint32_t w = west_of(j > 0 ? s[j - 1] : 0u, s[j]);
-            const uint32_t e = east_of(s[j], j + 1 < kFineN ? s[j + 1] : 0u);
+            const uint32_t e = east_of(s[j], j + 1 < N ? s[j + 1] : 0u);
             const RowSum r = row_sum(w, s[j], e);
             h0[j] = r.h0;
             h1[j] = r.h1;
@@ -303,7 +302,7 @@ __device__ __forceinline__ void fine_generations(uint32_t (&s)[kFineN], int gens
             e1[j] = r.e1;
         }
 #pragma unroll
-        for (int j = 0; j < kFineN; ++j) {
+        for (int j = 0; j < N; ++j) {
             const uint32_t a0 = from_lower_lane(h0[j]), a1 = from_lower_lane(h1[j]);  // row above
             const uint32_t b0 = from_upper_lane(h0[j]), b1 = from_upper_lane(h1[j]);  // row below
             uint32_t v = next_state(a0, a1, e0[j], e1[j], b0, b1, s[j]);
@@ -313,82 +312,81 @@ __device__ __forceinline__ void fine_generations(uint32_t (&s)[kFineN], int gens
     }
 }
 
-// Wavefront i of the launch advances tile list[i] (i < *count), or tile i
-// (i < dense_tiles) when list is null, by gens generations. With tracking
-// (tr.list_next set), a tile whose cells changed appends the tiles of its
-// 3 x 3 neighbourhood to the next block's list: a tile joins once, when its
-// mark goes from 0 to 1, and a wavefront reserves its entries with one atomic
-// add. A wavefront clears the mark of the listed tile it advances (its list
-// is done with it), and the launch resets the counter of the list after next.
-__global__ __launch_bounds__(64 * kFineWaves) void fine_tile_kernel(const uint32_t* __restrict__ src,
-                                                                    uint32_t* __restrict__ dst, FineGeo g, int gens,
-                                                                    const int* __restrict__ list,
-                                                                    const int* __restrict__ count, int dense_tiles,
-                                                                    FineTrack tr) {
-    if (tr.count_clear && blockIdx.x == 0 && threadIdx.x == 0) *tr.count_clear = 0;
-    const int slot = blockIdx.x * kFineWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (slot >= (list ? *count : dense_tiles)) return;  // whole wavefronts leave; there are no barriers
-    const int tile = list ? __builtin_amdgcn_readfirstlane(list[slot]) : slot;
+// Advances one tile by gens generations. With tracking (tr.list_next set), a
+// tile whose cells changed appends the tiles of its 3 x 3 neighbourhood to
+// the next block's list: a tile joins once, when its mark goes from 0 to 1,
+// and the wavefront reserves its entries with one atomic add. The wavefront
+// clears the mark of a listed tile it advances (its list is done with it).
+template <int DW>
+__device__ __forceinline__ void fine_tile(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                          const FineGeo& g, int gens, int tile, bool listed, const FineTrack& tr) {
+    constexpr int N = DW + 2;
     const int lane = threadIdx.x & 63;
     const int tx = tile % g.tiles_x, ty = tile / g.tiles_x;
-    const int x0 = tx * kFineDW - 1;                 // first dword held: the west halo dword
+    const int x0 = tx * DW - 1;                      // first dword held: the west halo dword
     const int y0 = ty * g.tile_rows - g.halo_rows;   // row of lane 0
     const int y = y0 + lane;
     const bool row_in = y >= 0 && y < g.y_rows;
     const uint32_t* row = src + static_cast<size_t>(row_in ? y : 0) * g.x_dw;
 
-    uint32_t s[kFineN], colmask[kFineN];
+    uint32_t s[N], colmask[N];
 #pragma unroll
-    for (int j = 0; j < kFineN; ++j) {
+    for (int j = 0; j < N; ++j) {
         const int gx = x0 + j;
         colmask[j] = (gx >= 0 && gx < g.x_dw) ? ~0u : 0u;
     }
-    // the tile's own dwords: two 16-byte loads when all eight lie in the row
-    // and rows start on 16-byte boundaries (x0 + 1 is a multiple of 8 dwords),
+    // the tile's own dwords: 16-byte loads when all DW lie in the row and rows
+    // start on 16-byte boundaries (x0 + 1 is a multiple of DW >= 4 dwords),
     // else one clamped load per dword
-    const bool full_x = x0 + 1 + kFineDW <= g.x_dw && (g.x_dw & 3) == 0;
+    const bool full_x = x0 + 1 + DW <= g.x_dw && (g.x_dw & 3) == 0;
     if (full_x) {
         const uint4* v = reinterpret_cast<const uint4*>(row + x0 + 1);
-        const uint4 lo = v[0], hi = v[1];
-        s[1] = lo.x; s[2] = lo.y; s[3] = lo.z; s[4] = lo.w;
-        s[5] = hi.x; s[6] = hi.y; s[7] = hi.z; s[8] = hi.w;
+#pragma unroll
+        for (int q = 0; q < DW / 4; ++q) {
+            const uint4 t = v[q];
+            s[1 + 4 * q] = t.x;
+            s[2 + 4 * q] = t.y;
+            s[3 + 4 * q] = t.z;
+            s[4 + 4 * q] = t.w;
+        }
     } else {
 #pragma unroll
-        for (int j = 1; j <= kFineDW; ++j) s[j] = row[colmask[j] ? x0 + j : 0] & colmask[j];
+        for (int j = 1; j <= DW; ++j) s[j] = row[colmask[j] ? x0 + j : 0] & colmask[j];
     }
     s[0] = row[colmask[0] ? x0 : 0] & colmask[0];
-    s[kFineN - 1] = row[colmask[kFineN - 1] ? x0 + kFineN - 1 : 0] & colmask[kFineN - 1];
+    s[N - 1] = row[colmask[N - 1] ? x0 + N - 1 : 0] & colmask[N - 1];
     const uint32_t rowmask = row_in ? ~0u : 0u;
 #pragma unroll
-    for (int j = 0; j < kFineN; ++j) s[j] &= rowmask;
+    for (int j = 0; j < N; ++j) s[j] &= rowmask;
 
-    uint32_t before[kFineDW];
+    uint32_t before[DW];
 #pragma unroll
-    for (int d = 0; d < kFineDW; ++d) before[d] = s[1 + d];
+    for (int d = 0; d < DW; ++d) before[d] = s[1 + d];
 
-    const bool edge = y0 < 0 || y0 + 64 > g.y_rows || x0 < 0 || x0 + kFineN > g.x_dw;  // wave-uniform
-    if (edge) fine_generations<true>(s, gens, rowmask, colmask);
-    else fine_generations<false>(s, gens, rowmask, colmask);
+    const bool edge = y0 < 0 || y0 + 64 > g.y_rows || x0 < 0 || x0 + N > g.x_dw;  // wave-uniform
+    if (edge) fine_generations<DW, true>(s, gens, rowmask, colmask);
+    else fine_generations<DW, false>(s, gens, rowmask, colmask);
 
     const bool own = lane >= g.halo_rows && lane < g.halo_rows + g.tile_rows && row_in;
     bool diff = false;
 #pragma unroll
-    for (int d = 0; d < kFineDW; ++d) diff |= before[d] != s[1 + d];
+    for (int d = 0; d < DW; ++d) diff |= before[d] != s[1 + d];
     if (own) {
         uint32_t* out = dst + static_cast<size_t>(y) * g.x_dw;
         if (full_x) {
             uint4* v = reinterpret_cast<uint4*>(out + x0 + 1);
-            v[0] = make_uint4(s[1], s[2], s[3], s[4]);
-            v[1] = make_uint4(s[5], s[6], s[7], s[8]);
+#pragma unroll
+            for (int q = 0; q < DW / 4; ++q)
+                v[q] = make_uint4(s[1 + 4 * q], s[2 + 4 * q], s[3 + 4 * q], s[4 + 4 * q]);
         } else {
 #pragma unroll
-            for (int j = 1; j <= kFineDW; ++j)
+            for (int j = 1; j <= DW; ++j)
                 if (colmask[j]) out[x0 + j] = s[j];
         }
     }
     if (!tr.list_next) return;
     if (lane == 0) {
-        if (list) tr.mark_cur[tile] = 0u;
+        if (listed) tr.mark_cur[tile] = 0u;
         atomicAdd(&tr.wg_active[blockIdx.x], 1ull);
     }
     if (__ballot(own && diff) == 0) return;
@@ -408,6 +406,26 @@ __global__ __launch_bounds__(64 * kFineWaves) void fine_tile_kernel(const uint32
     if (lane == 0) base = atomicAdd(tr.count_next, __popcll(m));
     base = __shfl(base, 0);
     if (add) tr.list_next[base + __popcll(m & ((1ull << lane) - 1))] = n;
+}
+
+// Wavefronts take slots i, i + stride, ... of tile list[0 .. *count), or of
+// tiles 0 .. dense_tiles - 1 when list is null (the grid is at most what the
+// device holds at once, so few wavefronts find no slot). The launch resets
+// the counter of the list after next.
+template <int DW>
+__global__ __launch_bounds__(64 * kFineWaves) void fine_tile_kernel(const uint32_t* __restrict__ src,
+                                                                    uint32_t* __restrict__ dst, FineGeo g, int gens,
+                                                                    const int* __restrict__ list,
+                                                                    const int* __restrict__ count, int dense_tiles,
+                                                                    FineTrack tr) {
+    if (tr.count_clear && blockIdx.x == 0 && threadIdx.x == 0) *tr.count_clear = 0;
+    const int n = list ? *count : dense_tiles;
+    const int stride = static_cast<int>(gridDim.x) * kFineWaves;
+    for (int slot = blockIdx.x * kFineWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); slot < n;
+         slot += stride) {
+        const int tile = list ? __builtin_amdgcn_readfirstlane(list[slot]) : slot;
+        fine_tile<DW>(src, dst, g, gens, tile, list != nullptr, tr);
+    }
 }
 
 // Batched grids: workgroup i advances work item i (a tile of one grid).
@@ -710,13 +728,32 @@ hipError_t launch_tiles(const TileGeometry& g, const uint32_t* src, uint32_t* ds
     return hipErrorInvalidValue;
 }
 
+static const void* fine_function(int dw) {
+    return dw == 4 ? reinterpret_cast<const void*>(&fine_tile_kernel<4>)
+                   : reinterpret_cast<const void*>(&fine_tile_kernel<8>);
+}
+
 hipError_t launch_fine_tiles(const FineGeo& g, const uint32_t* src, uint32_t* dst, int gens, const int* list,
                              const int* count, const FineTrack& track, hipStream_t stream) {
     const int n = g.tiles_x * g.tiles_y;
-    const unsigned blocks = static_cast<unsigned>(fine_workgroups(g));
-    hipLaunchKernelGGL(fine_tile_kernel, dim3(blocks), dim3(64 * kFineWaves), 0, stream, src, dst, g, gens, list, count,
-                       n, track);
+    const dim3 grid(static_cast<unsigned>(fine_workgroups(g))), block(64 * kFineWaves);
+    if (g.tile_dwords == 4)
+        hipLaunchKernelGGL(fine_tile_kernel<4>, grid, block, 0, stream, src, dst, g, gens, list, count, n, track);
+    else if (g.tile_dwords == 8)
+        hipLaunchKernelGGL(fine_tile_kernel<8>, grid, block, 0, stream, src, dst, g, gens, list, count, n, track);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
+}
+
+int fine_resident_workgroups(int tile_dwords) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fine_function(tile_dwords), 64 * kFineWaves, 0) !=
+            hipSuccess)
+        return 0;
+    return cus * per_cu;
 }
 
 hipError_t launch_items(const TileShape& shape, const BatchGrid* grids, const WorkItem* items, int n_items,

@@ -61,22 +61,29 @@ hipError_t launch_tiles(const TileGeometry& g, const uint32_t* src, uint32_t* ds
 
 TileGeo to_device_geo(const TileGeometry& g, int64_t x_dw, int64_t y_rows);
 
-// Change tracking over small tiles: one wavefront per tile of kFineDW dwords
-// (256 cells) x tile_rows rows, lane l holding row y0 + l of the tile's rows
-// and halo_rows (= T) halo rows above and below: tile_rows = 64 - 2T.
-constexpr int kFineDW = 8;      // dwords of a tile's row
-constexpr int kFineWaves = 4;   // wavefronts (tiles) per workgroup
+// Change tracking over small tiles: one wavefront per tile of tile_dwords
+// (4 or 8: 128 or 256 cells) x tile_rows rows, lane l holding row y0 + l of
+// the tile's rows and halo_rows (= T) halo rows above and below:
+// tile_rows = 64 - 2T.
+constexpr int kFineWaves = 4;      // wavefronts (tiles) per workgroup
+constexpr int kFineDefaultDW = 4;  // tile width in dwords when gol_config.lane_dwords is 0
 constexpr int kFineMaxSteps = 21;  // T <= tile_rows: a tile is at least as tall as T generations reach
 
 struct FineGeo {
     int x_dw, y_rows;
+    int tile_dwords;  // 4 or 8
     int tiles_x, tiles_y;
-    int tile_rows;  // 64 - 2 * halo_rows
-    int halo_rows;  // T: generations per block
+    int tile_rows;    // 64 - 2 * halo_rows
+    int halo_rows;    // T: generations per block
+    int workgroups;   // launch size: min(tiles / kFineWaves, resident workgroups)
 };
 
-// Workgroups of a change-tracking launch (kFineWaves tiles each).
-inline int fine_workgroups(const FineGeo& g) { return (g.tiles_x * g.tiles_y + kFineWaves - 1) / kFineWaves; }
+// Workgroups of a change-tracking launch (kFineWaves wavefronts each).
+inline int fine_workgroups(const FineGeo& g) { return g.workgroups; }
+
+// Workgroups the change-tracking kernel for tiles of tile_dwords can hold
+// resident at once on the current device (0 if unknown).
+int fine_resident_workgroups(int tile_dwords);
 
 // Change-tracking state of one block (all null: no tracking).
 struct FineTrack {

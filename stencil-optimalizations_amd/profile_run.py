@@ -5,7 +5,8 @@
   python3 profile_run.py tile     16384^2 random, 800 generations with the temporal-tile kernel
                                   (100 launches of 8 generations, persistent launch switched off)
   python3 profile_run.py plain    16384^2 random, 100 generations with the plain kernel
-  python3 profile_run.py active   16384^2 mostly-stable patches, 2400 generations, change tracking
+  python3 profile_run.py active   16384^2 mostly-stable patches, 20000 generations, change tracking
+                                  (the bench's mostly-stable configuration)
   python3 profile_run.py batch    256 mixed grids (sides 64..2048, 100..500 generations), one batch run
   python3 profile_run.py copy     calibration: a torch device copy of 256 MiB, 10 times
                                   (reads and writes exactly 256 MiB per dispatch)
@@ -51,9 +52,9 @@ def main(what: str) -> None:
         w = golhip.fill_patches(16384, 16384, 42, 256, 64, 0.35)
         with golhip.Grid(16384, 16384, kernel=golhip.KERNEL_ACTIVE, device=0) as g:
             g.upload(w)
-            ms = g.step_timed(2400)
+            ms = g.step_timed(20000)
             info = g.info()
-        print(f"active: 2400 generations, {info.launches} launches, {ms:.2f} ms GPU, "
+        print(f"active: 20000 generations, {info.launches} launches, {ms:.2f} ms GPU, "
               f"{info.active_tiles_total} active tile-blocks of {info.tiles * info.tracked_blocks}", flush=True)
     elif what == "batch":
         rng = np.random.default_rng(3)

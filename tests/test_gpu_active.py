@@ -40,25 +40,30 @@ def test_patches_against_naive(steps):
         assert info.tracked_blocks == gens // steps
 
 
-# Tiles are 256 cells x (64 - 2T) rows: partial last tile columns and rows,
-# odd dword counts (rows-32), grids shorter than a wavefront, T = 1 and 21.
-@pytest.mark.parametrize("x,y,steps,word_bits", [
-    (32, 1, 12, 32), (96, 5, 3, 32), (2112, 70, 12, 64), (288, 131, 21, 32), (4128, 300, 1, 32),
-    (640, 64, 20, 64), (256, 22, 21, 64), (1344, 257, 7, 64),
+# Tiles are 32 * lane_dwords cells x (64 - 2T) rows: partial last tile
+# columns and rows, odd dword counts (rows-32), grids shorter than a
+# wavefront, T = 1 and 21, both tile widths.
+@pytest.mark.parametrize("x,y,steps,word_bits,dw", [
+    (32, 1, 12, 32, 8), (96, 5, 3, 32, 8), (2112, 70, 12, 64, 8), (288, 131, 21, 32, 8), (4128, 300, 1, 32, 8),
+    (640, 64, 20, 64, 8), (256, 22, 21, 64, 8), (1344, 257, 7, 64, 8),
+    (32, 3, 12, 32, 4), (160, 41, 12, 32, 4), (2112, 70, 12, 64, 4), (4096, 300, 16, 64, 4), (1344, 257, 21, 64, 4),
 ])
-def test_ragged_sizes_against_naive(x, y, steps, word_bits):
+def test_ragged_sizes_against_naive(x, y, steps, word_bits, dw):
     c = ol.bernoulli_cells(x, y, x * 7 + y, 0.4)
     gens = 3 * steps + 2
-    note("active ragged", x, y, "steps", steps, "word_bits", word_bits, "gens", gens)
-    out, info = run(ol.pack(c, word_bits), x, y, gens, word_bits=word_bits, temporal_steps=steps)
+    note("active ragged", x, y, "steps", steps, "word_bits", word_bits, "lane_dwords", dw, "gens", gens)
+    out, info = run(ol.pack(c, word_bits), x, y, gens, word_bits=word_bits, temporal_steps=steps, lane_dwords=dw)
     assert np.array_equal(ol.unpack(out, x, y, word_bits), ol.naive_run(c, gens))
-    assert info.tiles == -(-x // 256) * -(-y // (64 - 2 * steps))
+    assert info.lane_dwords == dw
+    assert info.tiles == -(-x // (32 * dw)) * -(-y // (64 - 2 * steps))
     assert info.tracked_blocks == gens // steps
 
 
-def test_block_length_limit():
+def test_tile_limits():
     with pytest.raises(golhip.GolError):
         golhip.Grid(2048, 2048, kernel=golhip.KERNEL_ACTIVE, temporal_steps=22)
+    with pytest.raises(golhip.GolError):
+        golhip.Grid(2048, 2048, kernel=golhip.KERNEL_ACTIVE, lane_dwords=2)
 
 
 def test_skips_tiles_on_a_stable_grid():
