@@ -106,6 +106,30 @@ def leg_mostly_stable() -> dict:
     }
 
 
+TUNE_SIDES = (64, 128, 256, 512, 1024, 2048)
+
+
+def autotune(enabled: bool) -> dict:
+    """Measured tile shape and round depth per size class for the square sides of
+    the batched workload (gol_batch_autotune); they apply to every batch planned
+    afterwards in this process (servers keep the built-in choices)."""
+    if not enabled:
+        golhip.batch_tuning_reset()
+        return {"enabled": False}
+    t0 = time.perf_counter()
+    res = golhip.batch_autotune([(s, s) for s in TUNE_SIDES], device=0)
+    return {
+        "enabled": True,
+        "seconds": time.perf_counter() - t0,
+        "choices": {f"{r['x_cells']}x{r['y_cells']}": {
+            "shape": [r["lane_dwords"], r["band_rows"], r["waves"]],
+            "generations_per_round": r["generations_per_round"],
+            "candidates": r["candidates"],
+            "tuned_cell_updates_per_s": r["tuned_cell_updates_per_s"],
+            "builtin_cell_updates_per_s": r["builtin_cell_updates_per_s"]} for r in res},
+    }
+
+
 def leg_batched(n_grids: int, plain_sample: int) -> dict:
     """4096 uneven grids in one batch vs the same grids as plain per-grid launches
     (one launch per generation; a seeded sample of grids is run that way)."""
@@ -166,21 +190,16 @@ def leg_batched(n_grids: int, plain_sample: int) -> dict:
     }
 
 
-def leg_served(n_jobs: int, load: float, throughput_hint: float) -> dict:
-    """Poisson arrivals of jobs (side log-uniform [128, 8192], p U[0.1, 0.5],
-    generations log-uniform [100, 50000]) into the continuous-batching server at
-    a rate that loads the GPU to `load` of `throughput_hint` cell-updates/s."""
+def serve(specs, inputs, paced: bool):
+    """Submit the jobs (at their arrival times when paced, else all at once) to a
+    fresh server and wait for every result; returns (results, latencies, queue
+    times, wall seconds, server stats)."""
     import threading
 
-    probe = workloads.served_stream(n_jobs, 1.0)
-    mean_work = sum(s.cell_updates for s in probe) / len(probe)
-    rate = load * throughput_hint / mean_work
-    specs = workloads.served_stream(n_jobs, rate)
-    inputs = workloads.make_inputs(specs)
+    n_jobs = len(specs)
     lat = [0.0] * n_jobs
     queue_ms = [0.0] * n_jobs
     ids = [None] * n_jobs
-    submitted = threading.Event()
     with golhip.Server(device=0, hbm_budget_bytes=32 << 30) as srv:
         # warm-up: one small job through the whole path
         srv.wait(srv.submit(golhip.fill_random(256, 256, 1), 256, 256, 10))
@@ -192,12 +211,11 @@ def leg_served(n_jobs: int, load: float, throughput_hint: float) -> dict:
             try:
                 for i, s in enumerate(specs):
                     delay = t0 + s.arrival_s - time.perf_counter()
-                    if delay > 0:
+                    if paced and delay > 0:
                         time.sleep(delay)
                     ids[i] = srv.submit(inputs[i], s.side, s.side, s.generations)
             except Exception as e:  # surfaced by the waiting loop below
                 failure.append(e)
-            submitted.set()
 
         th = threading.Thread(target=submitter)
         th.start()
@@ -214,7 +232,28 @@ def leg_served(n_jobs: int, load: float, throughput_hint: float) -> dict:
         th.join()
         wall = time.perf_counter() - t0
         stats = srv.stats()
-    total = sum(s.cell_updates for s in specs)
+    return results, lat, queue_ms, wall, stats
+
+
+def leg_served(n_jobs: int, load: float) -> dict:
+    """Poisson arrivals of jobs (side log-uniform [128, 8192], p U[0.1, 0.5],
+    generations log-uniform [100, 50000]) into the continuous-batching server at
+    `load` of its capacity: the server's throughput on the same jobs submitted
+    all at once, measured first."""
+    probe = workloads.served_stream(n_jobs, 1.0)
+    inputs = workloads.make_inputs(probe)
+    total = sum(s.cell_updates for s in probe)
+    sat_results, _, _, sat_wall, _ = serve(probe, inputs, paced=False)
+    capacity = total / sat_wall
+    rate = load * capacity / (total / n_jobs)
+    specs = workloads.served_stream(n_jobs, rate)  # the same jobs, arrivals at `rate`
+    assert all(a.side == b.side and a.generations == b.generations for a, b in zip(specs, probe))
+    results, lat, queue_ms, wall, stats = serve(specs, inputs, paced=True)
+    # every job: the same result in the two different mixes
+    for i in range(n_jobs):
+        if not np.array_equal(results[i], sat_results[i]):
+            raise SystemExit(f"served job {i} differs between the paced and the all-at-once runs")
+    del sat_results
     # same result alone as in the mix, on a seeded sample of jobs
     rng = np.random.Generator(np.random.PCG64(5))
     sample = sorted(rng.choice(n_jobs, size=min(12, n_jobs), replace=False).tolist())
@@ -227,9 +266,10 @@ def leg_served(n_jobs: int, load: float, throughput_hint: float) -> dict:
                 raise SystemExit(f"served job {i} differs from the same job run alone")
     lat_sorted = sorted(lat)
     return {
-        "workload": f"{n_jobs} jobs, Poisson arrivals at {rate:.1f}/s (target load {load:.2f} of "
-                    f"{throughput_hint:.2e} cell-updates/s), side log-uniform [128, 8192] (x64), "
-                    f"p U[0.1,0.5], generations log-uniform [100, 50000]",
+        "workload": f"{n_jobs} jobs, Poisson arrivals at {rate:.1f}/s (load {load:.2f} of the server's "
+                    f"capacity {capacity:.2e} cell-updates/s, measured with every job submitted at once), "
+                    f"side log-uniform [128, 8192] (x64), p U[0.1,0.5], generations log-uniform [100, 50000]",
+        "capacity_cell_updates_per_s": capacity,
         "cell_updates": total,
         "wall_s": wall,
         "cell_updates_per_s": total / wall,
@@ -240,6 +280,7 @@ def leg_served(n_jobs: int, load: float, throughput_hint: float) -> dict:
         "rounds": stats.rounds,
         "resident_bytes_peak": stats.resident_bytes_peak,
         "sample_bit_exact_alone_vs_served": len(sample),
+        "all_jobs_identical_paced_vs_all_at_once": True,
     }
 
 
@@ -260,6 +301,8 @@ def main() -> int:
     ap.add_argument("--lane-dwords", type=int, default=0)
     ap.add_argument("--band-rows", type=int, default=0)
     ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched and served legs with the built-in tile choices instead of measured ones")
     ap.add_argument("--no-persistent", action="store_true",
                     help="one launch per block of generations instead of the persistent slab launch")
     ap.add_argument("--kernel", default="temporal", choices=["temporal", "plain"])
@@ -386,13 +429,14 @@ def main() -> int:
         log("mostly-stable leg ...")
         result["mostly_stable"] = leg_mostly_stable()
     if "batched" in legs:
+        log("autotuning batch size classes ...")
+        result["autotune"] = autotune(not args.no_autotune)
         log("batched leg ...")
         result["batched"] = leg_batched(args.batch_grids, args.plain_sample)
         result["batched_vs_plain_speedup"] = result["batched"]["speedup_vs_plain_per_grid"]
     if "served" in legs:
         log("served leg ...")
-        hint = result.get("batched", {}).get("batched_cell_updates_per_s", 2.5e13)
-        result["served"] = leg_served(args.served_jobs, args.served_load, hint)
+        result["served"] = leg_served(args.served_jobs, args.served_load)
         result["latency_p50_ms"] = result["served"]["latency_p50_ms"]
         result["latency_p99_ms"] = result["served"]["latency_p99_ms"]
     if not args.no_cpu_baseline:

@@ -179,6 +179,42 @@ int gol_batch_fetch(gol_batch* b, uint32_t grid_id, void* words, size_t n_words)
 int gol_batch_get_info(gol_batch* b, gol_batch_info* info);
 void gol_batch_destroy(gol_batch* b);
 
+/* ---- Autotuner for batches and servers (no reference counterpart: the
+ * reference picks kernel parameters by hand per run, ExperimentParams
+ * thread_block_size / temporal_steps, experiment_params.hpp:21-65). For each
+ * representative grid size it times every compiled tile shape (dwords per
+ * lane 1/2/4/8, i.e. 32- to 256-bit lane words; rows per wavefront;
+ * wavefronts per workgroup) and, for grids split into tiles, R, 3R/4, R/2,
+ * 3R/8, R/4 and R/8 generations per round, on a batch of synthetic grids of
+ * that size, and
+ * installs the fastest for the size class (power-of-two classes of the row
+ * width and the row count). The choices apply to every batch planned
+ * afterwards in the process, until gol_batch_tuning_reset. Servers keep the
+ * built-in choices: the measured ones favour a batch's throughput with
+ * shallower rounds, and a served job's latency grows with its rounds. ---- */
+typedef struct gol_tune_config {
+    int32_t device;              /* HIP device ordinal; -1 = current device */
+    uint32_t round_generations;  /* R, as in gol_batch_config (1..32); 0 = 32 */
+    uint64_t generations;        /* generations each measured grid advances; 0 = 4 R */
+    uint64_t cells_per_measure;  /* cells in one measured batch (grids of the size); 0 = 2^28 */
+} gol_tune_config;
+
+typedef struct gol_tune_entry {
+    uint64_t x_cells, y_cells;          /* in: a representative grid of the size class */
+    uint32_t lane_dwords, band_rows, waves;  /* out: the chosen tile shape */
+    uint32_t generations_per_round;     /* out: for split grids; 0 when one tile spans the grid */
+    uint32_t candidates;                /* out: choices timed */
+    uint32_t grids_measured;            /* out: grids per measured batch */
+    double tuned_cell_updates_per_s;    /* out: rate of the chosen shape */
+    double builtin_cell_updates_per_s;  /* out: rate of the built-in choice */
+} gol_tune_entry;
+
+/* Tune n_entries size classes (device memory: 3 x cells_per_measure / 8 bytes
+ * at a time). */
+int gol_batch_autotune(const gol_tune_config* config, gol_tune_entry* entries, uint32_t n_entries);
+/* Forget every measured choice: batches use the built-in table and cost model. */
+int gol_batch_tuning_reset(void);
+
 /* ---- Server: continuous batching of simulation jobs (no reference
  * counterpart). A scheduler thread inside the library advances all resident
  * jobs in rounds; between rounds it admits queued jobs (arrival order) whose

@@ -1,8 +1,9 @@
 // Batched execution of many independent grids (no reference counterpart: the
 // reference runs one grid per process, src/infrastructure/experiment_manager.hpp:235-270).
 //
-// Every grid gets a tile shape from its size class (autotuner table, or a
-// measured choice from gol_batch_tune) and its own ping-pong buffers in one
+// Every grid gets a tile shape from its size class (a measured choice from
+// gol_batch_autotune, else the built-in table and cost model) and its own
+// ping-pong buffers in one
 // device arena. Work is cut into rounds; in each round every unfinished grid
 // contributes one work item per tile, advancing that tile by the grid's
 // per-round generation count:
@@ -50,30 +51,50 @@ TileShape batch_default_shape(int64_t x_dw, int64_t y_rows) {
     return default_shape(x_dw, y_rows);
 }
 
-// Size classes of the measured autotuner: (lane width class, row class).
+// Size classes of the measured autotuner: power-of-two classes of the row
+// width in dwords and of the row count.
 struct SizeClass {
     int x_class, y_class;
     bool operator<(const SizeClass& o) const { return x_class != o.x_class ? x_class < o.x_class : y_class < o.y_class; }
 };
-static SizeClass size_class(int64_t x_dw, int64_t y_rows) {
-    int xc = x_dw <= 64 ? 0 : (x_dw <= 128 ? 1 : (x_dw <= 256 ? 2 : 3));
-    int yc = y_rows <= 64 ? 0 : (y_rows <= 128 ? 1 : (y_rows <= 512 ? 2 : 3));
-    return SizeClass{xc, yc};
+static int ceil_log2(int64_t v) {
+    int c = 0;
+    while ((int64_t{1} << c) < v) ++c;
+    return c;
 }
+static SizeClass size_class(int64_t x_dw, int64_t y_rows) { return SizeClass{ceil_log2(x_dw), ceil_log2(y_rows)}; }
+
+// A batch choice: tile shape and, for grids split into tiles, generations per
+// round (the halo depth; one-tile grids advance a chunk instead).
+struct BatchChoice {
+    TileShape shape;
+    int steps;
+};
 
 static std::mutex g_tune_mu;
-static std::map<SizeClass, TileShape> g_tuned;  // process-wide measured choices
+static std::map<SizeClass, BatchChoice> g_tuned;  // process-wide measured choices
+// A choice the autotuner is measuring; overrides everything in this thread.
+static thread_local const BatchChoice* t_forced = nullptr;
+// Set while the autotuner measures the built-in choice: measured choices are ignored.
+static thread_local bool t_untuned = false;
 
-TileShape batch_shape(int64_t x_dw, int64_t y_rows) {
+static bool tuned_choice(int64_t x_dw, int64_t y_rows, BatchChoice* c) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tuned.find(size_class(x_dw, y_rows));
-    if (it != g_tuned.end()) return it->second;
+    if (it == g_tuned.end()) return false;
+    *c = it->second;
+    return true;
+}
+
+TileShape batch_shape(int64_t x_dw, int64_t y_rows) {
+    BatchChoice c;
+    if (!t_untuned && tuned_choice(x_dw, y_rows, &c)) return c.shape;
     return batch_default_shape(x_dw, y_rows);
 }
 
 void set_tuned_shape(int64_t x_dw, int64_t y_rows, const TileShape& s) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tuned[size_class(x_dw, y_rows)] = s;
+    g_tuned[size_class(x_dw, y_rows)] = BatchChoice{s, 0};
 }
 
 // Cost per generation of advancing a whole grid split into tiles, in CU
@@ -89,12 +110,43 @@ static double split_cost_per_generation(const TileShape& s, const TileGeometry& 
     return compute + memory / steps;
 }
 
-int plan_batch_grid(int64_t x_dw, int64_t y_rows, int round_generations, TileShape* shape, TileGeometry* geom,
-                    int* per_round) {
+// The given choice, if it applies to the grid: one-tile shapes advance a
+// chunk per round, split shapes `steps` (<= round_generations) generations.
+static bool plan_choice(const BatchChoice& c, int64_t x_dw, int64_t y_rows, int round_generations, int big_tile_rows,
+                        TileShape* shape, TileGeometry* geom, int* per_round) {
+    const char* why = nullptr;
+    const TileShape& s = c.shape;
+    if (!shape_compiled(s)) return false;
+    if (x_dw <= 64 * s.lane_dwords && y_rows <= s.band_rows * s.waves) {
+        if (!plan_tiles(s, x_dw, y_rows, 1, geom, &why)) return false;
+        *shape = s;
+        *per_round = chunk_generations(s, round_generations, big_tile_rows);
+        return true;
+    }
+    const int steps = std::min(c.steps > 0 ? c.steps : round_generations, round_generations);
+    if (!plan_tiles(s, x_dw, y_rows, steps, geom, &why)) return false;
+    *shape = s;
+    *per_round = steps;
+    return true;
+}
+
+int plan_batch_grid(int64_t x_dw, int64_t y_rows, int round_generations, bool use_tuned, TileShape* shape,
+                    TileGeometry* geom, int* per_round) {
     constexpr int kBigTileRows = 512;
     const char* why = nullptr;
+    // a choice being measured, then a measured choice for the size class
+    if (t_forced) {
+        if (plan_choice(*t_forced, x_dw, y_rows, round_generations, kBigTileRows, shape, geom, per_round))
+            return GOL_OK;
+        return fail(GOL_ERR_RUNTIME, "tile shape under test does not fit the grid");
+    }
+    const bool tuned_ok = use_tuned && !t_untuned;
+    BatchChoice tuned;
+    if (tuned_ok && tuned_choice(x_dw, y_rows, &tuned) && tuned.steps > 0 &&
+        plan_choice(tuned, x_dw, y_rows, round_generations, kBigTileRows, shape, geom, per_round))
+        return GOL_OK;
     // grids that fit one tile of their size class need no halo
-    const TileShape table = batch_shape(x_dw, y_rows);
+    const TileShape table = tuned_ok ? batch_shape(x_dw, y_rows) : batch_default_shape(x_dw, y_rows);
     if (x_dw <= 64 * table.lane_dwords && y_rows <= table.band_rows * table.waves) {
         *shape = table;
         *per_round = chunk_generations(table, round_generations, kBigTileRows);
@@ -184,7 +236,7 @@ int build(gol_batch* b) {
     const int T = b->cfg.round_generations ? static_cast<int>(b->cfg.round_generations) : 32;
     size_t off = 0;
     for (auto& g : b->grids) {
-        int rc = plan_batch_grid(g.x_dw, g.y_rows, T, &g.shape, &g.geom, &g.chunk);
+        int rc = plan_batch_grid(g.x_dw, g.y_rows, T, true, &g.shape, &g.geom, &g.chunk);
         if (rc) return rc;
         g.offset = off;
         const size_t n = static_cast<size_t>(g.x_dw * g.y_rows);
@@ -436,6 +488,152 @@ void gol_batch_destroy(gol_batch* b) {
         if (b->stream) (void)hipStreamDestroy(b->stream);
     }
     delete b;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Autotuner: for each representative grid size, time every compiled tile
+// shape (and, for grids split into tiles, generations per round R, 3R/4, R/2,
+// 3R/8, R/4 and R/8)
+// on a batch of synthetic grids of that size filling the device, and install
+// the fastest for the size class. The built-in choice is timed too, for the
+// report. Measurements use graph replays timed with HIP events.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Measure {
+    double ms = 0.0;
+    uint64_t cell_updates = 0;
+};
+
+int measure_batch(int device, int round_generations, uint64_t x, uint64_t y, int count, uint64_t generations,
+                  const std::vector<uint32_t>& words, Measure* m) {
+    gol_batch_config bc{};
+    bc.device = device;
+    bc.round_generations = static_cast<uint32_t>(round_generations);
+    bc.use_graph = 1;
+    gol_batch* b = nullptr;
+    int rc = gol_batch_create(&bc, &b);
+    if (rc) return rc;
+    for (int i = 0; i < count && !rc; ++i) rc = gol_batch_add(b, x, y, 32, generations, words.data(), nullptr);
+    double ms = 0.0;
+    if (!rc) rc = gol_batch_run(b, &ms);  // plans, uploads, records the graph
+    if (!rc) rc = gol_batch_run(b, &ms);  // the timed replay
+    gol_batch_info info{};
+    if (!rc) rc = gol_batch_get_info(b, &info);
+    gol_batch_destroy(b);
+    if (rc) return rc;
+    m->ms = ms;
+    m->cell_updates = info.cell_updates;
+    return GOL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gol_batch_autotune(const gol_tune_config* config, gol_tune_entry* entries, uint32_t n_entries) {
+    clear_error();
+    if (n_entries > 0 && !entries) return fail(GOL_ERR_INVALID_ARGUMENT, "entries is null");
+    gol_tune_config cfg{};
+    cfg.device = -1;
+    if (config) cfg = *config;
+    const int R = cfg.round_generations ? static_cast<int>(cfg.round_generations) : 32;
+    if (R > 32) return fail(GOL_ERR_INVALID_ARGUMENT, "round_generations must be at most 32 (the one-lane x halo)");
+    const uint64_t target = cfg.cells_per_measure ? cfg.cells_per_measure : (uint64_t{1} << 28);
+    const uint64_t gens = cfg.generations ? cfg.generations : 4ull * static_cast<uint64_t>(R);
+    for (uint32_t k = 0; k < n_entries; ++k) {
+        const uint64_t x = entries[k].x_cells, y = entries[k].y_cells;
+        if (x == 0 || y == 0 || x % 32 != 0 || x / 32 > (1u << 20) || y > (1u << 20))
+            return fail(GOL_ERR_INVALID_ARGUMENT, "entry %u: grid %llux%llu is not a valid size", k,
+                        static_cast<unsigned long long>(x), static_cast<unsigned long long>(y));
+    }
+    int device = cfg.device;
+    hipError_t e;
+    if (device < 0 && (e = hipGetDevice(&device)) != hipSuccess) return fail_hip(e, "hipGetDevice");
+
+    for (uint32_t k = 0; k < n_entries; ++k) {
+        gol_tune_entry& out = entries[k];
+        const uint64_t x = out.x_cells, y = out.y_cells;
+        const int64_t x_dw = static_cast<int64_t>(x / 32), y_rows = static_cast<int64_t>(y);
+        const int count = static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(4096, target / (x * y))));
+        std::vector<uint32_t> words(static_cast<size_t>(x_dw * y_rows));
+        int rc = gol_fill_bernoulli(words.data(), x, y, 32, 20240601ull + k, 0.35);
+        if (rc) return rc;
+
+        std::vector<BatchChoice> cands;
+        for (int i = 0; i < kNumTileShapes; ++i) {
+            const TileShape& s = kTileShapes[i];
+            if (x_dw <= 64 * s.lane_dwords && y_rows <= s.band_rows * s.waves) {
+                cands.push_back(BatchChoice{s, 1});
+                continue;
+            }
+            int last = 0;
+            for (int steps : {R, 3 * R / 4, R / 2, 3 * R / 8, R / 4, R / 8}) {
+                if (steps < 1 || steps == last) continue;
+                last = steps;
+                TileGeometry g;
+                const char* why = nullptr;
+                if (plan_tiles(s, x_dw, y_rows, steps, &g, &why)) cands.push_back(BatchChoice{s, steps});
+            }
+        }
+
+        // the built-in choice
+        Measure base;
+        t_untuned = true;
+        rc = measure_batch(device, R, x, y, count, gens, words, &base);
+        t_untuned = false;
+        if (rc) return rc;
+        TileShape base_shape;
+        TileGeometry base_geom;
+        int base_round = 0;
+        t_untuned = true;
+        rc = plan_batch_grid(x_dw, y_rows, R, true, &base_shape, &base_geom, &base_round);
+        t_untuned = false;
+        if (rc) return rc;
+
+        BatchChoice best{base_shape, base_round};
+        double best_rate = base.cell_updates / (base.ms * 1e-3);
+        uint32_t measured = 0;
+        for (const auto& c : cands) {
+            Measure m;
+            t_forced = &c;
+            rc = measure_batch(device, R, x, y, count, gens, words, &m);
+            t_forced = nullptr;
+            if (rc) {
+                clear_error();
+                continue;  // this shape cannot run the grid: not a candidate
+            }
+            ++measured;
+            const double rate = m.cell_updates / (m.ms * 1e-3);
+            if (rate > best_rate) {
+                best_rate = rate;
+                best = c;
+            }
+        }
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tuned[size_class(x_dw, y_rows)] = BatchChoice{best.shape, std::max(best.steps, 1)};
+        }
+        out.lane_dwords = static_cast<uint32_t>(best.shape.lane_dwords);
+        out.band_rows = static_cast<uint32_t>(best.shape.band_rows);
+        out.waves = static_cast<uint32_t>(best.shape.waves);
+        const bool one_tile = x_dw <= 64 * best.shape.lane_dwords && y_rows <= best.shape.band_rows * best.shape.waves;
+        out.generations_per_round = one_tile ? 0u : static_cast<uint32_t>(best.steps);
+        out.candidates = measured;
+        out.grids_measured = static_cast<uint32_t>(count);
+        out.tuned_cell_updates_per_s = best_rate;
+        out.builtin_cell_updates_per_s = base.cell_updates / (base.ms * 1e-3);
+    }
+    return GOL_OK;
+}
+
+int gol_batch_tuning_reset(void) {
+    clear_error();
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned.clear();
+    return GOL_OK;
 }
 
 }  // extern "C"

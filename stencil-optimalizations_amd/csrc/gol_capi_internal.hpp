@@ -16,16 +16,17 @@ const char* last_error();
 
 TileShape default_shape(int64_t x_dw, int64_t y_rows);
 
-// Batch size classes: the measured choice if gol_batch_tune found one, else the table.
+// Batch size classes: the measured choice if gol_batch_autotune found one, else the table.
 TileShape batch_default_shape(int64_t x_dw, int64_t y_rows);
 TileShape batch_shape(int64_t x_dw, int64_t y_rows);
 void set_tuned_shape(int64_t x_dw, int64_t y_rows, const TileShape& s);
 
 // Tile shape, geometry and generations per round of one grid in a batch or a
-// server: grids split into tiles advance round_generations per round, grids
-// that fit one tile advance a chunk of about the same cost.
-int plan_batch_grid(int64_t x_dw, int64_t y_rows, int round_generations, TileShape* shape, TileGeometry* geom,
-                    int* per_round);
+// server: with use_tuned, the autotuner's choice for the size class when there
+// is one; otherwise grids split into tiles advance round_generations per
+// round, and grids that fit one tile advance a chunk of about the same cost.
+int plan_batch_grid(int64_t x_dw, int64_t y_rows, int round_generations, bool use_tuned, TileShape* shape,
+                    TileGeometry* geom, int* per_round);
 bool shape_compiled(const TileShape& s);
 
 }  // namespace golhip

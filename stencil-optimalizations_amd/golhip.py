@@ -40,9 +40,14 @@ EXPORTED_SYMBOLS = (
     "gol_batch_get_info", "gol_batch_destroy",
     "gol_server_create", "gol_server_submit", "gol_server_poll", "gol_server_wait",
     "gol_server_get_stats", "gol_server_destroy",
+    "gol_batch_autotune", "gol_batch_tuning_reset",
 )
 
 JOB_QUEUED, JOB_RESIDENT, JOB_DRAINING, JOB_DONE, JOB_FAILED = range(5)
+
+# tile shapes compiled into the library: (dwords per lane, rows per wavefront, wavefronts)
+TILE_SHAPES = ((8, 8, 8), (8, 10, 8), (8, 5, 16), (8, 4, 16), (4, 8, 8), (4, 8, 16), (2, 16, 8), (1, 8, 8),
+               (1, 16, 8), (1, 32, 16))
 
 
 class GolError(RuntimeError):
@@ -104,6 +109,26 @@ class GolBatchInfo(ctypes.Structure):
         ("launches", ctypes.c_uint64),
         ("device_bytes", ctypes.c_uint64),
         ("cell_updates", ctypes.c_uint64),
+    ]
+
+
+class GolTuneConfig(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int32),
+        ("round_generations", ctypes.c_uint32),
+        ("generations", ctypes.c_uint64),
+        ("cells_per_measure", ctypes.c_uint64),
+    ]
+
+
+class GolTuneEntry(ctypes.Structure):
+    _fields_ = [
+        ("x_cells", ctypes.c_uint64), ("y_cells", ctypes.c_uint64),
+        ("lane_dwords", ctypes.c_uint32), ("band_rows", ctypes.c_uint32), ("waves", ctypes.c_uint32),
+        ("generations_per_round", ctypes.c_uint32),
+        ("candidates", ctypes.c_uint32), ("grids_measured", ctypes.c_uint32),
+        ("tuned_cell_updates_per_s", ctypes.c_double),
+        ("builtin_cell_updates_per_s", ctypes.c_double),
     ]
 
 
@@ -177,6 +202,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.gol_batch_get_info.argtypes = [vp, ctypes.POINTER(GolBatchInfo)]
     lib.gol_batch_destroy.argtypes = [vp]
     lib.gol_batch_destroy.restype = None
+    lib.gol_batch_autotune.argtypes = [ctypes.POINTER(GolTuneConfig), ctypes.POINTER(GolTuneEntry), ctypes.c_uint32]
+    lib.gol_batch_autotune.restype = ctypes.c_int
+    lib.gol_batch_tuning_reset.argtypes = []
+    lib.gol_batch_tuning_reset.restype = ctypes.c_int
     lib.gol_server_create.argtypes = [ctypes.POINTER(GolServerConfig), ctypes.POINTER(vp)]
     lib.gol_server_submit.argtypes = [vp, u64, u64, ctypes.c_uint32, u64, vp, ctypes.POINTER(u64)]
     lib.gol_server_poll.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_int)]
@@ -327,6 +356,28 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def batch_autotune(sizes, device: int = -1, round_generations: int = 0, generations: int = 0,
+                   cells_per_measure: int = 0) -> list:
+    """gol_batch_autotune over representative (x_cells, y_cells) sizes; returns one
+    dict per size with the chosen shape and the measured rates."""
+    lib = load_library()
+    sizes = list(sizes)
+    entries = (GolTuneEntry * max(1, len(sizes)))()
+    for e, (x, y) in zip(entries, sizes):
+        e.x_cells, e.y_cells = x, y
+    cfg = GolTuneConfig(device, round_generations, generations, cells_per_measure)
+    _check(lib.gol_batch_autotune(ctypes.byref(cfg), entries, len(sizes)))
+    return [dict(x_cells=e.x_cells, y_cells=e.y_cells, lane_dwords=e.lane_dwords, band_rows=e.band_rows,
+                 waves=e.waves, generations_per_round=e.generations_per_round, candidates=e.candidates,
+                 grids_measured=e.grids_measured, tuned_cell_updates_per_s=e.tuned_cell_updates_per_s,
+                 builtin_cell_updates_per_s=e.builtin_cell_updates_per_s) for e in entries[:len(sizes)]]
+
+
+def batch_tuning_reset() -> None:
+    """gol_batch_tuning_reset: back to the built-in table and cost model."""
+    _check(load_library().gol_batch_tuning_reset())
 
 
 class Server:

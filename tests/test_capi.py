@@ -49,6 +49,14 @@ def test_config_errors_before_device(kw, code):
     assert e.value.code == code
 
 
+@pytest.mark.parametrize("size,rounds", [((100, 64), 0), ((0, 64), 0), ((2048, 2048), 33)])
+def test_autotune_errors_before_device(size, rounds):
+    # invalid sizes and round depths fail in validation, before any HIP call
+    with pytest.raises(golhip.GolError) as e:
+        golhip.batch_autotune([size], round_generations=rounds)
+    assert e.value.code == golhip.GOL_ERR_INVALID_ARGUMENT
+
+
 def test_error_message_names_the_multiple():
     with pytest.raises(golhip.GolError) as e:
         golhip.Grid(96, 8, word_bits=64)

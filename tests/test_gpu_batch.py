@@ -81,3 +81,34 @@ def test_word_bits_32_in_batch():
         gid = b.add(ol.pack(c, 32), 96, 50, 77, word_bits=32)
         b.run()
         assert np.array_equal(ol.unpack(b.fetch(gid), 96, 50, 32), ol.naive_run(c, 77))
+
+
+def test_autotune_installs_measured_choices():
+    """The autotuner times every compiled shape per size class and installs the
+    fastest; batches planned afterwards use it and stay bit-exact."""
+    sizes = [(2048, 2048), (640, 512), (128, 128), (4096, 96)]
+    try:
+        note("autotune", sizes)
+        res = golhip.batch_autotune(sizes, round_generations=16, generations=64, cells_per_measure=1 << 25)
+        assert len(res) == len(sizes)
+        for r in res:
+            assert r["candidates"] >= 1
+            assert r["tuned_cell_updates_per_s"] >= r["builtin_cell_updates_per_s"] > 0
+            assert (r["lane_dwords"], r["band_rows"], r["waves"]) in {tuple(s) for s in golhip.TILE_SHAPES}
+            one_tile = (r["x_cells"] <= 2048 * r["lane_dwords"]
+                        and r["y_cells"] <= r["band_rows"] * r["waves"])
+            assert (r["generations_per_round"] == 0) == one_tile
+            if not one_tile:
+                assert 1 <= r["generations_per_round"] <= 16
+        specs = [(2048, 2048, 70, 0.4, 11), (640, 512, 150, 0.3, 12), (128, 128, 400, 0.5, 13),
+                 (4096, 96, 33, 0.45, 14), (1984, 1600, 50, 0.35, 15)]
+        with golhip.Batch(round_generations=16) as b:
+            for x, y, gens, p, seed in specs:
+                b.add(golhip.fill_bernoulli(x, y, seed, p), x, y, gens)
+            note("batch after tuning")
+            b.run()
+            for i, (x, y, gens, p, seed) in enumerate(specs):
+                ref = ol.bitpar_run(golhip.fill_bernoulli(x, y, seed, p), x, y, gens)
+                assert np.array_equal(b.fetch(i), ref), (x, y, gens)
+    finally:
+        golhip.batch_tuning_reset()
