@@ -92,11 +92,6 @@ TileShape batch_shape(int64_t x_dw, int64_t y_rows) {
     return batch_default_shape(x_dw, y_rows);
 }
 
-void set_tuned_shape(int64_t x_dw, int64_t y_rows, const TileShape& s) {
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tuned[size_class(x_dw, y_rows)] = BatchChoice{s, 0};
-}
-
 // Cost per generation of advancing a whole grid split into tiles, in CU
 // microseconds, when many grids share the GPU (no wave quantization): the
 // compute of every tile row (about 13.5 VALU ops per dword at 64 lanes per
@@ -142,7 +137,7 @@ int plan_batch_grid(int64_t x_dw, int64_t y_rows, int round_generations, bool us
     }
     const bool tuned_ok = use_tuned && !t_untuned;
     BatchChoice tuned;
-    if (tuned_ok && tuned_choice(x_dw, y_rows, &tuned) && tuned.steps > 0 &&
+    if (tuned_ok && tuned_choice(x_dw, y_rows, &tuned) &&
         plan_choice(tuned, x_dw, y_rows, round_generations, kBigTileRows, shape, geom, per_round))
         return GOL_OK;
     // grids that fit one tile of their size class need no halo

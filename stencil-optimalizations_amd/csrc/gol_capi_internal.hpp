@@ -19,7 +19,6 @@ TileShape default_shape(int64_t x_dw, int64_t y_rows);
 // Batch size classes: the measured choice if gol_batch_autotune found one, else the table.
 TileShape batch_default_shape(int64_t x_dw, int64_t y_rows);
 TileShape batch_shape(int64_t x_dw, int64_t y_rows);
-void set_tuned_shape(int64_t x_dw, int64_t y_rows, const TileShape& s);
 
 // Tile shape, geometry and generations per round of one grid in a batch or a
 // server: with use_tuned, the autotuner's choice for the size class when there
