@@ -130,6 +130,36 @@ def autotune(enabled: bool) -> dict:
     }
 
 
+def leg_spacefiller() -> dict:
+    """The reference's published workload (cluster-experiments/generate-final-measurements.py:33,
+    hyper-params-measurements/_scripts/rel-work--part_2.sh: spacefiller[8192,8192] on 16384^2):
+    the dense persistent launch against change tracking, 20000 generations each."""
+    gens = 20000
+    expr = "spacefiller[8192, 8192]"
+    w = golhip.fill_lexicon(X, Y, expr)
+    f = fixture("spacefiller-16384", gens)
+    rates = {}
+    for label, kernel in (("dense", golhip.KERNEL_TEMPORAL), ("change_tracking", golhip.KERNEL_ACTIVE)):
+        with golhip.Grid(X, Y, kernel=kernel, device=0) as g:
+            g.upload(w)
+            g.step_timed(24)  # warm-up
+            g.upload(w)
+            ms = g.step_timed(gens)
+            out = g.download()
+            info = g.info()
+        exact = hashlib.sha256(out.tobytes()).hexdigest() == f["sha256"] if f else None
+        if exact is False:
+            raise SystemExit(f"spacefiller: the {label} result differs from the oracle fixture")
+        rates[label] = {"ms": ms, "cell_updates_per_s": X * Y * gens / (ms / 1e3), "bit_exact_vs_oracle": exact}
+        if kernel == golhip.KERNEL_ACTIVE:
+            rates[label]["active_tile_fraction"] = info.active_tiles_total / (info.tiles * info.tracked_blocks)
+    return {
+        "workload": f"16384x16384 empty with {expr}, {gens} generations (the reference's published pattern)",
+        **rates,
+        "speedup_change_tracking_vs_dense": rates["dense"]["ms"] / rates["change_tracking"]["ms"],
+    }
+
+
 def leg_batched(n_grids: int, plain_sample: int) -> dict:
     """4096 uneven grids in one batch vs the same grids as plain per-grid launches
     (one launch per generation; a seeded sample of grids is run that way)."""
@@ -308,8 +338,8 @@ def main() -> int:
     ap.add_argument("--kernel", default="temporal", choices=["temporal", "plain"])
     ap.add_argument("--cpu-sample-generations", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--legs", default="mostly_stable,batched,served",
-                    help="extra workloads reported beside the headline: mostly_stable,batched,served")
+    ap.add_argument("--legs", default="mostly_stable,spacefiller,batched,served",
+                    help="extra workloads reported beside the headline: mostly_stable,spacefiller,batched,served")
     ap.add_argument("--batch-grids", type=int, default=4096)
     ap.add_argument("--plain-sample", type=int, default=48)
     ap.add_argument("--served-jobs", type=int, default=400)
@@ -428,6 +458,9 @@ def main() -> int:
     if "mostly_stable" in legs:
         log("mostly-stable leg ...")
         result["mostly_stable"] = leg_mostly_stable()
+    if "spacefiller" in legs:
+        log("spacefiller leg ...")
+        result["spacefiller"] = leg_spacefiller()
     if "batched" in legs:
         log("autotuning batch size classes ...")
         result["autotune"] = autotune(not args.no_autotune)

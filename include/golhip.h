@@ -280,12 +280,18 @@ void gol_server_destroy(gol_server* s);
  *  gol_fill_patches: empty grid with n_patches squares of side `patch`, corner
  *    (rng() % (X-patch+1), rng() % (Y-patch+1)) then Bernoulli(p) cells from the
  *    same mt19937_64(seed) stream, later patches overwriting earlier ones
- *    (no reference counterpart: the mostly-stable change-tracking workload). */
+ *    (no reference counterpart: the mostly-stable change-tracking workload);
+ *  gol_fill_lexicon: the reference's pattern-expression loader
+ *    (src/infrastructure/gol-lexicon/lexicon.hpp:92-186): an empty grid and
+ *    "name[x, y]; ..." Life Lexicon patterns (block, blinker, beehive, glider,
+ *    gosper-glider-gun, spacefiller), each pattern's rectangle written at its
+ *    top-left corner (x, y), cells beyond the grid skipped. */
 int gol_fill_random(void* words, uint64_t x_cells, uint64_t y_cells, uint32_t word_bits, uint64_t seed);
 int gol_fill_bernoulli(void* words, uint64_t x_cells, uint64_t y_cells, uint32_t word_bits, uint64_t seed,
                        double p);
 int gol_fill_patches(void* words, uint64_t x_cells, uint64_t y_cells, uint32_t word_bits, uint64_t seed,
                      uint32_t n_patches, uint32_t patch, double p);
+int gol_fill_lexicon(void* words, uint64_t x_cells, uint64_t y_cells, uint32_t word_bits, const char* expression);
 
 /* Message of the calling thread's last failed call ("" if none). */
 const char* gol_last_error(void);

@@ -12,6 +12,9 @@
 //                       [0, Y-patch] as rng() % (range+1)); each patch's cells are
 //                       Bernoulli(p) drawn from the same stream in row-major order; later
 //                       patches overwrite earlier ones.
+//   gol_fill_lexicon    the reference's pattern-expression loader
+//                       (src/infrastructure/gol-lexicon/lexicon.hpp:92-186), shared with
+//                       the driver through host/include/stencils/lexicon.hpp.
 
 #include <cstdint>
 #include <cstring>
@@ -19,6 +22,7 @@
 #include <vector>
 
 #include "../../include/golhip.h"
+#include "../host/include/stencils/lexicon.hpp"
 #include "gol_capi_internal.hpp"
 
 using namespace golhip;
@@ -99,6 +103,23 @@ int gol_fill_patches(void* words, uint64_t x_cells, uint64_t y_cells, uint32_t w
                 set_cell(words, x_cells, word_bits, px + dx, py + dy, unit_double(rng) < p);
             }
         }
+    }
+    return GOL_OK;
+}
+
+int gol_fill_lexicon(void* words, uint64_t x_cells, uint64_t y_cells, uint32_t word_bits, const char* expression) {
+    clear_error();
+    int rc = check_dims(words, x_cells, y_cells, word_bits);
+    if (rc) return rc;
+    if (!expression) return fail(GOL_ERR_INVALID_ARGUMENT, "expression is null");
+    std::memset(words, 0, x_cells / 8 * y_cells);
+    try {
+        static const infrastructure::PatternTable table;
+        infrastructure::place_patterns(expression, x_cells, y_cells, table, [&](std::size_t x, std::size_t y, bool alive) {
+            set_cell(words, x_cells, word_bits, x, y, alive);
+        });
+    } catch (const std::exception& e) {
+        return fail(GOL_ERR_INVALID_ARGUMENT, "%s", e.what());
     }
     return GOL_OK;
 }

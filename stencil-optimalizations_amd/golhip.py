@@ -35,7 +35,7 @@ KERNEL_ACTIVE = 3
 EXPORTED_SYMBOLS = (
     "gol_create", "gol_upload", "gol_upload_device", "gol_step", "gol_step_timed", "gol_download", "gol_sync",
     "gol_device_state", "gol_get_stream", "gol_get_info", "gol_destroy", "gol_last_error",
-    "gol_version", "gol_fill_random", "gol_fill_bernoulli", "gol_fill_patches",
+    "gol_version", "gol_fill_random", "gol_fill_bernoulli", "gol_fill_patches", "gol_fill_lexicon",
     "gol_batch_create", "gol_batch_add", "gol_batch_run", "gol_batch_reset", "gol_batch_fetch",
     "gol_batch_get_info", "gol_batch_destroy",
     "gol_server_create", "gol_server_submit", "gol_server_poll", "gol_server_wait",
@@ -192,6 +192,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     u64 = ctypes.c_uint64
     lib.gol_fill_random.argtypes = [vp, u64, u64, ctypes.c_uint32, u64]
     lib.gol_fill_bernoulli.argtypes = [vp, u64, u64, ctypes.c_uint32, u64, ctypes.c_double]
+    lib.gol_fill_lexicon.argtypes = [vp, u64, u64, ctypes.c_uint32, ctypes.c_char_p]
     lib.gol_fill_patches.argtypes = [vp, u64, u64, ctypes.c_uint32, u64, ctypes.c_uint32,
                                      ctypes.c_uint32, ctypes.c_double]
     lib.gol_batch_create.argtypes = [ctypes.POINTER(GolBatchConfig), ctypes.POINTER(vp)]
@@ -221,7 +222,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         getattr(lib, name).restype = ctypes.c_int
     for name in ("gol_create", "gol_upload", "gol_upload_device", "gol_step", "gol_step_timed", "gol_download",
                  "gol_sync", "gol_device_state", "gol_get_stream", "gol_get_info",
-                 "gol_fill_random", "gol_fill_bernoulli", "gol_fill_patches"):
+                 "gol_fill_random", "gol_fill_bernoulli", "gol_fill_patches", "gol_fill_lexicon"):
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -459,6 +460,13 @@ def fill_patches(x_cells: int, y_cells: int, seed: int, n_patches: int, patch: i
     w = _words(x_cells, y_cells, word_bits)
     _check(load_library().gol_fill_patches(w.ctypes.data, x_cells, y_cells, word_bits, seed,
                                            n_patches, patch, p))
+    return w
+
+
+def fill_lexicon(x_cells: int, y_cells: int, expression: str, word_bits: int = 64) -> np.ndarray:
+    """gol_fill_lexicon: "name[x, y]; ..." Life Lexicon patterns on an empty grid."""
+    w = _words(x_cells, y_cells, word_bits)
+    _check(load_library().gol_fill_lexicon(w.ctypes.data, x_cells, y_cells, word_bits, expression.encode()))
     return w
 
 

@@ -12,6 +12,7 @@
 
 #include "experiment_params.hpp"
 #include "grid.hpp"
+#include "lexicon.hpp"
 
 namespace infrastructure {
 
@@ -45,79 +46,11 @@ class ZerosLoader : public Loader<Dims, ElementType> {
     }
 };
 
-// Named patterns in run-length form ("b" dead, "o" alive, "$" end of row, a
-// count before a symbol repeats it). The shapes are those of the Life Lexicon
-// entries the reference's workloads name.
-class PatternTable {
-   public:
-    using Pattern = std::vector<std::vector<char>>;
-
-    PatternTable() {
-        add("block", "2o$2o!");
-        add("blinker", "3o!");
-        add("beehive", "b2o$o2bo$b2o!");
-        add("glider", "3o$o$bo!");
-        add("gosper-glider-gun",
-            "24bo$22bobo$12b2o6b2o12b2o$11bo3bo4b2o12b2o$2o8bo5bo3b2o$2o8bo3bob2o4bobo$10bo5bo7bo$11bo3bo$12b2o!");
-    }
-    // Unknown names give an empty pattern, as the reference's lexicon does.
-    const Pattern& get(const std::string& name) const {
-        auto it = table_.find(name);
-        return it == table_.end() ? empty_ : it->second;
-    }
-    bool has(const std::string& name) const { return table_.count(name) != 0; }
-
-   private:
-    void add(const std::string& name, const std::string& rle) {
-        Pattern p(1);
-        int count = 0;
-        for (char ch : rle) {
-            if (std::isdigit(static_cast<unsigned char>(ch))) {
-                count = count * 10 + (ch - '0');
-                continue;
-            }
-            const int n = count ? count : 1;
-            count = 0;
-            if (ch == 'b' || ch == 'o') {
-                for (int i = 0; i < n; ++i) p.back().push_back(ch == 'o' ? 1 : 0);
-            } else if (ch == '$') {
-                for (int i = 0; i < n; ++i) p.emplace_back();
-            } else if (ch == '!') {
-                break;
-            }
-        }
-        table_[name] = p;
-    }
-    std::map<std::string, Pattern> table_;
-    Pattern empty_;
-};
-
-// "name[x, y]; name[x, y]; ..." — each pattern's top-left cell at (x, y);
-// cells beyond the grid are skipped (the reference writes them unchecked).
+// "name[x, y]; ..." expressions into a grid (lexicon.hpp).
 template <typename ElementType>
 void insert_patterns(Grid<2, ElementType>& grid, const std::string& expression, const PatternTable& table) {
-    std::string e;
-    for (char ch : expression)
-        if (!std::isspace(static_cast<unsigned char>(ch))) e.push_back(ch);
-    std::size_t i = 0;
-    while (i < e.size()) {
-        std::string name, xs, ys;
-        while (i < e.size() && e[i] != '[') name.push_back(e[i++]);
-        if (i >= e.size()) throw std::invalid_argument("pattern expression: missing '[' after " + name);
-        ++i;
-        while (i < e.size() && e[i] != ',') xs.push_back(e[i++]);
-        ++i;
-        while (i < e.size() && e[i] != ']') ys.push_back(e[i++]);
-        if (i >= e.size()) throw std::invalid_argument("pattern expression: missing ']' after " + name);
-        i += 1;
-        if (i < e.size() && e[i] == ';') ++i;
-        const std::size_t x0 = std::stoul(xs), y0 = std::stoul(ys);
-        const auto& p = table.get(name);
-        for (std::size_t y = 0; y < p.size(); ++y)
-            for (std::size_t x = 0; x < p[y].size(); ++x)
-                if (x0 + x < grid.template size_in<0>() && y0 + y < grid.template size_in<1>())
-                    grid[x0 + x][y0 + y] = static_cast<ElementType>(p[y][x]);
-    }
+    place_patterns(expression, grid.template size_in<0>(), grid.template size_in<1>(), table,
+                   [&](std::size_t x, std::size_t y, bool alive) { grid[x][y] = static_cast<ElementType>(alive); });
 }
 
 template <int Dims, typename ElementType>

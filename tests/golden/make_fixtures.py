@@ -59,6 +59,8 @@ BITPAR_CASES = [
     ("random-16384", 16384, 16384, "random", 42, {}, [1, 100, 1000, 10000]),
     ("patches-16384", 16384, 16384, "patches", 42, {"n_patches": 256, "patch": 64, "p": 0.35}, [100, 20000]),
     ("random-4096x8192", 4096, 8192, "bernoulli", 7, {"p": 0.4}, [500]),
+    # the reference's published workload (cluster-experiments/generate-final-measurements.py:33)
+    ("spacefiller-16384", 16384, 16384, "lexicon", 0, {"expression": "spacefiller[8192, 8192]"}, [1000, 20000]),
 ]
 
 
@@ -68,7 +70,45 @@ def input_cells(x, y, gen, seed, p):
     return ol.bernoulli_cells(x, y, seed, p)
 
 
+def bitpar_case(name, x, y, gen, seed, params, gens_list):
+    if gen == "random":
+        w = golhip.fill_random(x, y, seed)
+    elif gen == "bernoulli":
+        w = golhip.fill_bernoulli(x, y, seed, params["p"])
+    elif gen == "lexicon":
+        w = golhip.fill_lexicon(x, y, params["expression"])
+    else:
+        w = golhip.fill_patches(x, y, seed, params["n_patches"], params["patch"], params["p"])
+    w0 = w
+    done = 0
+    runs = []
+    for gens in sorted(gens_list):
+        t = time.time()
+        w = ol.bitpar_run(w, x, y, gens - done)
+        done = gens
+        runs.append({
+            "name": name, "x": x, "y": y, "generator": gen, "seed": seed, "params": params,
+            "generations": gens, "sha256_input": ol.sha256(w0), "sha256": ol.sha256(w),
+            "popcount": int(np.unpackbits(w.view(np.uint8)).sum()),
+        })
+        print(f"bitpar {name} gens={gens}: {time.time() - t:.1f}s", flush=True)
+    return runs
+
+
 def main() -> None:
+    path = os.path.join(HERE, "fixtures.json")
+    if len(sys.argv) > 2 and sys.argv[1] == "--only":
+        # recompute the named full-size cases and keep every other fixture
+        out = json.load(open(path))
+        names = set(sys.argv[2:])
+        out["bitpar_runs"] = [r for r in out["bitpar_runs"] if r["name"] not in names]
+        for case in BITPAR_CASES:
+            if case[0] in names:
+                out["bitpar_runs"] += bitpar_case(*case)
+        with open(path, "w") as f:
+            json.dump(out, f, indent=1)
+            f.write("\n")
+        return
     out = {
         "an5d_kats": AN5D_KATS,
         "mt19937_10000": 4123659995,
@@ -96,26 +136,9 @@ def main() -> None:
                 "generations": gens, "sha256": ol.sha256(words), "popcount": int(cur.sum()),
             })
             print(f"naive {x}x{y} {gen} seed={seed} gens={gens}: {time.time() - t:.1f}s", flush=True)
-    for name, x, y, gen, seed, params, gens_list in BITPAR_CASES:
-        if gen == "random":
-            w = golhip.fill_random(x, y, seed)
-        elif gen == "bernoulli":
-            w = golhip.fill_bernoulli(x, y, seed, params["p"])
-        else:
-            w = golhip.fill_patches(x, y, seed, params["n_patches"], params["patch"], params["p"])
-        w0 = w
-        done = 0
-        for gens in sorted(gens_list):
-            t = time.time()
-            w = ol.bitpar_run(w, x, y, gens - done)
-            done = gens
-            out["bitpar_runs"].append({
-                "name": name, "x": x, "y": y, "generator": gen, "seed": seed, "params": params,
-                "generations": gens, "sha256_input": ol.sha256(w0), "sha256": ol.sha256(w),
-                "popcount": int(np.unpackbits(w.view(np.uint8)).sum()),
-            })
-            print(f"bitpar {name} gens={gens}: {time.time() - t:.1f}s", flush=True)
-    with open(os.path.join(HERE, "fixtures.json"), "w") as f:
+    for case in BITPAR_CASES:
+        out["bitpar_runs"] += bitpar_case(*case)
+    with open(path, "w") as f:
         json.dump(out, f, indent=1)
         f.write("\n")
 

@@ -93,6 +93,30 @@ def test_fill_patches_places_patches():
     assert 0.25 < one.sum() / (64 * 64) < 0.45
 
 
+def test_fill_lexicon_spacefiller_matches_the_reference_table():
+    # the 27 x 27 spacefiller of the reference's pattern table
+    # (src/infrastructure/gol-lexicon/patterns.cpp:10325-10352): 187 live cells;
+    # the hash is of np.packbits over that matrix, row by row
+    c = ol.unpack(golhip.fill_lexicon(64, 64, "spacefiller[10, 20]"), 64, 64)
+    assert c.sum() == 187
+    ys, xs = np.nonzero(c)
+    assert (ys.min(), ys.max(), xs.min(), xs.max()) == (20, 46, 10, 36)
+    assert ol.sha256(np.packbits(c[20:47, 10:37])) == \
+        "85ae4a83ebbfa90bfd769602877f4972560597184999e53e472f134ef100de8f"
+
+
+def test_fill_lexicon_places_rectangles_and_clips():
+    c = ol.unpack(golhip.fill_lexicon(64, 64, "glider[60, 62]; block[0, 0]", word_bits=32), 64, 64, 32)
+    assert c.sum() == 8  # the block and the two rows of the glider inside the grid
+    assert c[0:2, 0:2].all() and c[62, 60:63].all() and c[63, 60] == 1
+    # a later pattern's dead cells overwrite earlier live ones
+    c = ol.unpack(golhip.fill_lexicon(64, 64, "block[0, 0]; beehive[0, 0]"), 64, 64)
+    assert c[0, 0] == 0 and c[0, 1] == 1 and c[1, 0] == 1
+    assert c.sum() == 6
+    with pytest.raises(golhip.GolError):
+        golhip.fill_lexicon(64, 64, "glider[1, 2")
+
+
 def test_python_pack_matches_oracle():
     c = ol.bernoulli_cells(192, 7, 3, 0.5)
     assert np.array_equal(golhip.pack_rows(c, 64).reshape(-1), ol.pack(c, 64))

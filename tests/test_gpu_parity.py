@@ -228,3 +228,23 @@ def test_persistent_used_for_16384_and_split_calls():
         for k in (1, 7, 8, 100, 300, 584):   # 1000 generations in uneven calls
             g.step(k)
         assert ol.sha256(g.download()) == f["sha256"]
+
+
+# the reference's published workload: one spacefiller in the middle of a
+# 16384^2 grid (cluster-experiments/generate-final-measurements.py:33)
+@pytest.mark.parametrize("gens,kw", [
+    (1000, dict(kernel=golhip.KERNEL_PLAIN)),
+    (20000, dict(kernel=golhip.KERNEL_TEMPORAL)),
+    (1000, dict(kernel=golhip.KERNEL_TEMPORAL, flags=golhip.FLAG_NO_PERSISTENT)),
+    (20000, dict(kernel=golhip.KERNEL_ACTIVE)),
+    (20000, dict(kernel=golhip.KERNEL_ACTIVE, lane_dwords=8)),
+])
+def test_spacefiller_fixture(gens, kw):
+    f = [r for r in FIX["bitpar_runs"] if r["name"] == "spacefiller-16384" and r["generations"] == gens][0]
+    w = golhip.fill_lexicon(16384, 16384, f["params"]["expression"])
+    assert ol.sha256(w) == f["sha256_input"]
+    note("spacefiller", gens, kw)
+    with golhip.Grid(16384, 16384, **kw) as g:
+        g.upload(w)
+        g.step(gens)
+        assert ol.sha256(g.download()) == f["sha256"]
