@@ -64,10 +64,24 @@ enum gol_kernel {
                                 (reference: cuda-naive-local-one-cell/cuda_local_one_cell.cu:53-242) */
 };
 
+/* Bit layout of the words passed to gol_upload / gol_upload_device /
+ * gol_download (reference: src/algorithms/_shared/bitwise/bit_modes.hpp). The
+ * engine advances the row layout; other layouts are converted on the device
+ * at upload and download. */
+enum gol_layout {
+    GOL_LAYOUT_ROWS = 0,    /* word (wx, y) at y*(X/W)+wx, bit k = cell (W*wx+k, y); W = 32 or 64
+                               (FullyPackedRows, bit_modes.hpp:127-163) */
+    GOL_LAYOUT_COLUMNS = 1, /* word (x, wy) at wy*X+x, bit k = cell (x, W*wy+k); W = 16, 32 or 64,
+                               Y a multiple of W (BitColumns, bit_modes.hpp:10-32) */
+    GOL_LAYOUT_TILES = 2    /* 4x4 / 8x4 / 8x8 cells in a 16 / 32 / 64-bit word (wx, wy) at
+                               wy*(X/XB)+wx, cell (tx, ty) in bit W-1-(ty*XB+tx); Y a multiple of
+                               the tile height (BitTile, bit_modes.hpp:37-125) */
+};
+
 typedef struct gol_config {
-    uint64_t x_cells;        /* grid width in cells, a multiple of word_bits */
+    uint64_t x_cells;        /* grid width in cells, a multiple of 32 and of word_bits (rows) */
     uint64_t y_cells;        /* grid height in cells (rows) */
-    uint32_t word_bits;      /* 32 or 64 */
+    uint32_t word_bits;      /* 32 or 64 (rows); 16, 32 or 64 (columns, tiles) */
     uint32_t kernel;         /* enum gol_kernel */
     uint32_t temporal_steps; /* generations per temporal launch; 0 = chosen by the library
                                 (reference: ExperimentParams::temporal_steps,
@@ -77,6 +91,8 @@ typedef struct gol_config {
     uint32_t waves;          /* wavefronts per workgroup; 0 = chosen by the library */
     int32_t device;          /* HIP device ordinal; -1 = current device */
     uint32_t flags;          /* GOL_FLAG_* */
+    uint32_t layout;         /* enum gol_layout of the words crossing the boundary; X is a
+                                multiple of 32 in every layout */
 } gol_config;
 
 /* gol_config.flags */
@@ -91,7 +107,7 @@ typedef struct gol_info {
     uint32_t temporal_steps;  /* generations per launch (per block for the persistent launch) */
     uint32_t tiles;           /* workgroups per launch */
     uint32_t persistent;      /* 1: one cooperative launch per gol_step, slabs exchange halos in memory */
-    uint32_t reserved;
+    uint32_t layout;          /* enum gol_layout of uploads and downloads */
     uint64_t launches;        /* kernel launches issued so far */
     uint64_t generations;     /* generations enqueued so far (actually_performed_iterations) */
     uint64_t active_tiles_total; /* GOL_KERNEL_ACTIVE: tiles computed, summed over blocks (syncs) */

@@ -22,6 +22,13 @@
 //   oracle_pack_rows /      algorithms/_shared/bitwise/general_bit_grid.hpp:171-246
 //   oracle_unpack_rows      with the FullyPackedRows policy, bit_modes.hpp:127-163
 //                           (word (wx,y) at y*(X/W)+wx, bit k = cell W*wx+k)
+//   oracle_pack_layout /    general_bit_grid.hpp:171-246 for any of the bit
+//   oracle_unpack_layout    policies of bit_modes.hpp: rows (FullyPackedRows,
+//                           :127-163), columns (BitColumns, :10-32: X_BITS 1,
+//                           Y_BITS W, bit y % W) and tiles (BitTile, :37-125:
+//                           4x4 / 8x4 / 8x8 cells in 16 / 32 / 64 bits, first
+//                           cell in the most significant bit); word (wx, wy) at
+//                           wy * (X / X_BITS) + wx
 //   oracle_bitpar_run       a second, independent CPU engine (uint64 rows,
 //                           3x3-word neighbourhood, like cpu-bitwise-general
 //                           algorithms/cpu-bitwise-general/gol_cpu_bitwise_general.hpp:32-76
@@ -164,6 +171,81 @@ int oracle_unpack_rows(const void* words, uint64_t x_size, uint64_t y_size, uint
                                               : static_cast<const uint32_t*>(words)[y * wpr + wx];
             for (uint32_t k = 0; k < word_bits; ++k) {
                 cells[y * x_size + wx * word_bits + k] = (word >> k) & 1;
+            }
+        }
+    }
+    return 0;
+}
+
+// Bit policy of a layout (0 rows, 1 columns, 2 tiles) and word width:
+// cells per word in x and y, and whether the first cell takes the most
+// significant bit (BitTileCommon, bit_modes.hpp:37-56) or the least.
+static bool layout_policy(uint32_t layout, uint32_t w, uint32_t* xb, uint32_t* yb, bool* msb_first) {
+    if (w != 16 && w != 32 && w != 64) return false;
+    if (layout == 0) {
+        *xb = w, *yb = 1, *msb_first = false;
+    } else if (layout == 1) {
+        *xb = 1, *yb = w, *msb_first = false;
+    } else if (layout == 2) {
+        *xb = w == 16 ? 4 : 8, *yb = w == 64 ? 8 : 4, *msb_first = true;
+    } else {
+        return false;
+    }
+    return true;
+}
+
+static void store_word(void* words, uint64_t i, uint32_t w, uint64_t v) {
+    if (w == 64) static_cast<uint64_t*>(words)[i] = v;
+    else if (w == 32) static_cast<uint32_t*>(words)[i] = static_cast<uint32_t>(v);
+    else static_cast<uint16_t*>(words)[i] = static_cast<uint16_t>(v);
+}
+
+static uint64_t load_word(const void* words, uint64_t i, uint32_t w) {
+    if (w == 64) return static_cast<const uint64_t*>(words)[i];
+    if (w == 32) return static_cast<const uint32_t*>(words)[i];
+    return static_cast<const uint16_t*>(words)[i];
+}
+
+// general_bit_grid.hpp:217-246 (fill_grid): per word, the cells in y_bit, x_bit
+// order take successive bits from the policy's first mask.
+int oracle_pack_layout(const uint8_t* cells, uint64_t x_size, uint64_t y_size, uint32_t layout, uint32_t word_bits,
+                       void* words) {
+    uint32_t xb, yb;
+    bool msb;
+    if (!layout_policy(layout, word_bits, &xb, &yb, &msb) || x_size % xb || y_size % yb) return 1;
+    const uint64_t wx_n = x_size / xb;
+    for (uint64_t wy = 0; wy < y_size / yb; ++wy) {
+        for (uint64_t wx = 0; wx < wx_n; ++wx) {
+            uint64_t word = 0;
+            uint32_t bit = msb ? word_bits - 1 : 0;
+            for (uint32_t y_bit = 0; y_bit < yb; ++y_bit) {
+                for (uint32_t x_bit = 0; x_bit < xb; ++x_bit) {
+                    if (cells[(wy * yb + y_bit) * x_size + wx * xb + x_bit]) word |= uint64_t{1} << bit;
+                    bit = msb ? bit - 1 : bit + 1;
+                }
+            }
+            store_word(words, wy * wx_n + wx, word_bits, word);
+        }
+    }
+    return 0;
+}
+
+// general_bit_grid.hpp:171-200 (to_grid)
+int oracle_unpack_layout(const void* words, uint64_t x_size, uint64_t y_size, uint32_t layout, uint32_t word_bits,
+                         uint8_t* cells) {
+    uint32_t xb, yb;
+    bool msb;
+    if (!layout_policy(layout, word_bits, &xb, &yb, &msb) || x_size % xb || y_size % yb) return 1;
+    const uint64_t wx_n = x_size / xb;
+    for (uint64_t wy = 0; wy < y_size / yb; ++wy) {
+        for (uint64_t wx = 0; wx < wx_n; ++wx) {
+            const uint64_t word = load_word(words, wy * wx_n + wx, word_bits);
+            uint32_t bit = msb ? word_bits - 1 : 0;
+            for (uint32_t y_bit = 0; y_bit < yb; ++y_bit) {
+                for (uint32_t x_bit = 0; x_bit < xb; ++x_bit) {
+                    cells[(wy * yb + y_bit) * x_size + wx * xb + x_bit] = (word >> bit) & 1;
+                    bit = msb ? bit - 1 : bit + 1;
+                }
             }
         }
     }

@@ -122,6 +122,7 @@ struct gol_handle {
     int cur = 0;
     int64_t x_dw = 0, y_rows = 0;
     size_t n_dwords = 0;
+    void* staging = nullptr;  // words of a non-row layout on the device (n_dwords * 4 bytes)
     uint32_t kernel = GOL_KERNEL_TEMPORAL;
     uint32_t plain_dw = 1;
     TileShape shape{1, 16, 8};
@@ -152,12 +153,21 @@ namespace {
 
 int validate_config(const gol_config* c) {
     if (!c) return fail(GOL_ERR_INVALID_ARGUMENT, "config is null");
-    if (c->word_bits != 32 && c->word_bits != 64)
+    if (c->layout > GOL_LAYOUT_TILES) return fail(GOL_ERR_INVALID_ARGUMENT, "unknown layout %u", c->layout);
+    if (c->layout == GOL_LAYOUT_ROWS && c->word_bits != 32 && c->word_bits != 64)
         return fail(GOL_ERR_INVALID_ARGUMENT, "word_bits must be 32 or 64, got %u", c->word_bits);
+    if (c->layout != GOL_LAYOUT_ROWS && c->word_bits != 16 && c->word_bits != 32 && c->word_bits != 64)
+        return fail(GOL_ERR_INVALID_ARGUMENT, "word_bits must be 16, 32 or 64, got %u", c->word_bits);
     if (c->x_cells == 0 || c->y_cells == 0)
         return fail(GOL_ERR_INVALID_ARGUMENT, "grid dimensions must be positive");
-    if (c->x_cells % c->word_bits != 0)
-        return fail(GOL_ERR_INVALID_ARGUMENT, "Grid dimensions X must be a multiple of %u", c->word_bits);
+    const uint32_t x_multiple = c->layout == GOL_LAYOUT_ROWS ? c->word_bits : 32;
+    if (c->x_cells % x_multiple != 0)
+        return fail(GOL_ERR_INVALID_ARGUMENT, "Grid dimensions X must be a multiple of %u", x_multiple);
+    // words of the columns and tiles layouts span several rows
+    const uint32_t y_multiple =
+        c->layout == GOL_LAYOUT_COLUMNS ? c->word_bits : (c->layout == GOL_LAYOUT_TILES ? (c->word_bits == 64 ? 8 : 4) : 1);
+    if (c->y_cells % y_multiple != 0)
+        return fail(GOL_ERR_INVALID_ARGUMENT, "Grid dimensions Y must be a multiple of %u", y_multiple);
     if (c->x_cells / 32 > (1u << 26) || c->y_cells > (1u << 30) || (c->x_cells / 32) * c->y_cells > (1ull << 34))
         return fail(GOL_ERR_INVALID_ARGUMENT, "grid too large");
     if (c->kernel > GOL_KERNEL_ACTIVE) return fail(GOL_ERR_INVALID_ARGUMENT, "unknown kernel %u", c->kernel);
@@ -421,7 +431,8 @@ int gol_create(const gol_config* config, gol_handle** out) {
     if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreate(&h->ev_start)) != hipSuccess || (e = hipEventCreate(&h->ev_stop)) != hipSuccess ||
         (e = hipMalloc(&h->buf[0], bytes)) != hipSuccess || (e = hipMalloc(&h->buf[1], bytes)) != hipSuccess ||
-        (e = hipMemsetAsync(h->buf[0], 0, bytes, h->stream)) != hipSuccess) {
+        (e = hipMemsetAsync(h->buf[0], 0, bytes, h->stream)) != hipSuccess ||
+        (h->cfg.layout != GOL_LAYOUT_ROWS && (e = hipMalloc(&h->staging, bytes)) != hipSuccess)) {
         rc = fail_hip(e, "gol_create");
         gol_destroy(h);
         return rc;
@@ -467,7 +478,14 @@ int gol_upload(gol_handle* h, const void* words, size_t n_words) {
     const size_t want = h->n_dwords * 32 / h->cfg.word_bits;
     if (n_words != want) return fail(GOL_ERR_INVALID_ARGUMENT, "expected %zu words, got %zu", want, n_words);
     DeviceGuard g(h->device);
-    hipError_t e = hipMemcpyAsync(h->buf[h->cur], words, h->n_dwords * 4, hipMemcpyHostToDevice, h->stream);
+    hipError_t e;
+    if (h->cfg.layout != GOL_LAYOUT_ROWS) {
+        if ((e = hipMemcpyAsync(h->staging, words, h->n_dwords * 4, hipMemcpyHostToDevice, h->stream)) == hipSuccess)
+            e = layout_to_rows(static_cast<int>(h->cfg.layout), static_cast<int>(h->cfg.word_bits), h->staging,
+                               h->buf[h->cur], h->x_dw * 32, h->y_rows, h->stream);
+    } else {
+        e = hipMemcpyAsync(h->buf[h->cur], words, h->n_dwords * 4, hipMemcpyHostToDevice, h->stream);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return fail_hip(e, "gol_upload");
     h->flags_valid = false;  // a new state: every tile is active in the next block
@@ -480,7 +498,11 @@ int gol_upload_device(gol_handle* h, const void* device_words, size_t n_words) {
     const size_t want = h->n_dwords * 32 / h->cfg.word_bits;
     if (n_words != want) return fail(GOL_ERR_INVALID_ARGUMENT, "expected %zu words, got %zu", want, n_words);
     DeviceGuard g(h->device);
-    hipError_t e = hipMemcpyAsync(h->buf[h->cur], device_words, h->n_dwords * 4, hipMemcpyDeviceToDevice, h->stream);
+    hipError_t e =
+        h->cfg.layout != GOL_LAYOUT_ROWS
+            ? layout_to_rows(static_cast<int>(h->cfg.layout), static_cast<int>(h->cfg.word_bits), device_words,
+                             h->buf[h->cur], h->x_dw * 32, h->y_rows, h->stream)
+            : hipMemcpyAsync(h->buf[h->cur], device_words, h->n_dwords * 4, hipMemcpyDeviceToDevice, h->stream);
     if (e != hipSuccess) return fail_hip(e, "gol_upload_device");
     h->flags_valid = false;
     return GOL_OK;
@@ -519,7 +541,14 @@ int gol_download(gol_handle* h, void* words, size_t n_words) {
     DeviceGuard g(h->device);
     int rc = check_slab_error(h);
     if (rc) return rc;
-    hipError_t e = hipMemcpyAsync(words, h->buf[h->cur], h->n_dwords * 4, hipMemcpyDeviceToHost, h->stream);
+    hipError_t e = hipSuccess;
+    const void* from = h->buf[h->cur];
+    if (h->cfg.layout != GOL_LAYOUT_ROWS) {
+        e = rows_to_layout(static_cast<int>(h->cfg.layout), static_cast<int>(h->cfg.word_bits), h->buf[h->cur],
+                           h->staging, h->x_dw * 32, h->y_rows, h->stream);
+        from = h->staging;
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(words, from, h->n_dwords * 4, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return fail_hip(e, "gol_download");
     return GOL_OK;
@@ -567,7 +596,8 @@ int gol_get_info(gol_handle* h, gol_info* info) {
     }
     info->launches = h->launches;
     info->generations = h->generations;
-    info->device_bytes = 2 * h->n_dwords * 4;
+    info->device_bytes = (h->staging ? 3 : 2) * h->n_dwords * 4;
+    info->layout = h->cfg.layout;
     if (h->kernel == GOL_KERNEL_ACTIVE) {
         DeviceGuard g(h->device);
         std::vector<unsigned long long> per_wg(static_cast<size_t>(fine_workgroups(h->fine)));
@@ -592,6 +622,7 @@ void gol_destroy(gol_handle* h) {
         if (h->stream) (void)hipStreamSynchronize(h->stream);
         if (h->buf[0]) (void)hipFree(h->buf[0]);
         if (h->buf[1]) (void)hipFree(h->buf[1]);
+        if (h->staging) (void)hipFree(h->staging);
         if (h->marks) (void)hipFree(h->marks);
         if (h->lists) (void)hipFree(h->lists);
         if (h->counts) (void)hipFree(h->counts);

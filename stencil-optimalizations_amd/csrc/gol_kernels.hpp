@@ -126,6 +126,17 @@ struct SlabArgs {
 bool slab_capacity(const TileShape& shape, bool mask_cols, int* workgroups);
 hipError_t launch_slab(const TileShape& shape, const SlabArgs& args, hipStream_t stream);
 
+// Bit layouts of the C ABI besides rows (gol_config.layout; the values of
+// enum gol_layout): conversion from a layout's words to the row layout and
+// back, on the stream. word_bits 16, 32 or 64; X a multiple of 32; Y a
+// multiple of the word's rows (W for columns, 4 or 8 for tiles).
+constexpr int kLayoutColumns = 1;
+constexpr int kLayoutTiles = 2;
+hipError_t layout_to_rows(int layout, int word_bits, const void* src, uint32_t* rows, int64_t x_cells,
+                          int64_t y_rows, hipStream_t stream);
+hipError_t rows_to_layout(int layout, int word_bits, const uint32_t* rows, void* dst, int64_t x_cells,
+                          int64_t y_rows, hipStream_t stream);
+
 // Plain kernel: one generation per launch, global memory to global memory.
 // lane_dwords in {1,2,4,8}.
 hipError_t launch_plain(int lane_dwords, const uint32_t* src, uint32_t* dst, int64_t x_dw, int64_t y_rows,

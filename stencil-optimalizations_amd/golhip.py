@@ -68,10 +68,13 @@ class GolConfig(ctypes.Structure):
         ("waves", ctypes.c_uint32),
         ("device", ctypes.c_int32),
         ("flags", ctypes.c_uint32),
+        ("layout", ctypes.c_uint32),
     ]
 
 
 FLAG_NO_PERSISTENT = 1
+
+LAYOUT_ROWS, LAYOUT_COLUMNS, LAYOUT_TILES = 0, 1, 2
 
 
 class GolInfo(ctypes.Structure):
@@ -83,7 +86,7 @@ class GolInfo(ctypes.Structure):
         ("temporal_steps", ctypes.c_uint32),
         ("tiles", ctypes.c_uint32),
         ("persistent", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("layout", ctypes.c_uint32),
         ("launches", ctypes.c_uint64),
         ("generations", ctypes.c_uint64),
         ("active_tiles_total", ctypes.c_uint64),
@@ -239,16 +242,16 @@ class Grid:
 
     def __init__(self, x_cells: int, y_cells: int, word_bits: int = 64, kernel: int = KERNEL_AUTO,
                  temporal_steps: int = 0, lane_dwords: int = 0, band_rows: int = 0, waves: int = 0,
-                 device: int = -1, flags: int = 0):
+                 device: int = -1, flags: int = 0, layout: int = LAYOUT_ROWS):
         self.lib = load_library()
         cfg = GolConfig(x_cells, y_cells, word_bits, kernel, temporal_steps, lane_dwords, band_rows,
-                        waves, device, flags)
+                        waves, device, flags, layout)
         h = ctypes.c_void_p()
         _check(self.lib.gol_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h
         self.x_cells, self.y_cells, self.word_bits = x_cells, y_cells, word_bits
-        self.dtype = np.uint64 if word_bits == 64 else np.uint32
-        self.n_words = (x_cells // word_bits) * y_cells
+        self.dtype = {16: np.uint16, 32: np.uint32, 64: np.uint64}[word_bits]
+        self.n_words = x_cells * y_cells // word_bits
 
     def upload(self, words: np.ndarray) -> None:
         words = np.ascontiguousarray(words, dtype=self.dtype).reshape(-1)

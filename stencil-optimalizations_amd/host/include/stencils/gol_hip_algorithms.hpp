@@ -10,6 +10,8 @@
 //   gol-hip-local-32 / -64      change tracking over active tiles
 //                               (the role of cuda-naive-local-one-cell, cuda_local_one_cell.cu:53-242)
 //   gol-hip-auto-64             shape and depth chosen by the library's time model
+//   gol-hip-cols-16/32/64       the same engine fed in the reference's column
+//   gol-hip-tiles-16/32/64      and tile bit layouts (BitColumns, BitTile)
 //
 // Parameters read from ExperimentParams: temporal_steps (> 1: generations per
 // launch; 0 or 1: chosen by the library), thread_block_size (> 0: wavefronts
@@ -55,22 +57,42 @@ inline void check_gol(int rc) {
     }
 }
 
-// Row packing (FullyPackedRows, bit_modes.hpp:127-163): word (wx, y) at
-// y * (X / Bits) + wx, bit k = cell (Bits * wx + k, y); cells > 0 are alive.
-template <std::size_t Bits>
-struct RowWords {
-    using word_type = std::conditional_t<Bits == 64, std::uint64_t, std::uint32_t>;
+// Packing under the reference's bit policies (GeneralBitGrid fill_grid /
+// to_grid, general_bit_grid.hpp:171-246, with bit_modes.hpp):
+//   GOL_LAYOUT_ROWS    FullyPackedRows (:127-163): word (wx, y) at y * (X / Bits) + wx,
+//                      bit k = cell (Bits * wx + k, y);
+//   GOL_LAYOUT_COLUMNS BitColumns (:10-32): word (x, wy) at wy * X + x, bit k = cell (x, Bits * wy + k);
+//   GOL_LAYOUT_TILES   BitTile (:37-125): XB x YB cells (4x4, 8x4, 8x8 for 16, 32, 64 bits),
+//                      word (wx, wy) at wy * (X / XB) + wx, cell (tx, ty) in bit Bits-1-(ty*XB+tx).
+// Cells > 0 are alive.
+template <uint32_t Layout, std::size_t Bits>
+struct LayoutWords {
+    using word_type = std::conditional_t<Bits == 64, std::uint64_t,
+                                         std::conditional_t<Bits == 32, std::uint32_t, std::uint16_t>>;
+    static constexpr std::size_t XB = Layout == GOL_LAYOUT_ROWS ? Bits
+                                      : Layout == GOL_LAYOUT_COLUMNS ? 1
+                                                                     : (Bits == 16 ? 4 : 8);
+    static constexpr std::size_t YB = Layout == GOL_LAYOUT_ROWS ? 1
+                                      : Layout == GOL_LAYOUT_COLUMNS ? Bits
+                                                                     : (Bits == 64 ? 8 : 4);
+    static constexpr bool kMsbFirst = Layout == GOL_LAYOUT_TILES;
 
     template <typename cell_t>
     static std::vector<word_type> pack(const infrastructure::Grid<2, cell_t>& grid) {
         const std::size_t X = grid.template size_in<0>(), Y = grid.template size_in<1>();
-        if (X % Bits != 0)
-            throw std::invalid_argument("Grid dimensions X must be a multiple of " + std::to_string(Bits));
-        std::vector<word_type> words(X / Bits * Y, 0);
+        if (Y % YB != 0) throw std::invalid_argument("Grid dimensions Y must be a multiple of " + std::to_string(YB));
+        if (X % XB != 0) throw std::invalid_argument("Grid dimensions X must be a multiple of " + std::to_string(XB));
+        std::vector<word_type> words(X / XB * (Y / YB), 0);
         const cell_t* d = grid.data();
-        for (std::size_t y = 0; y < Y; ++y)
-            for (std::size_t x = 0; x < X; ++x)
-                if (d[y * X + x] > 0) words[y * (X / Bits) + x / Bits] |= word_type{1} << (x % Bits);
+        for (std::size_t wy = 0; wy < Y / YB; ++wy)
+            for (std::size_t wx = 0; wx < X / XB; ++wx) {
+                word_type w = 0;
+                std::size_t bit = kMsbFirst ? Bits - 1 : 0;
+                for (std::size_t yb = 0; yb < YB; ++yb)
+                    for (std::size_t xb = 0; xb < XB; ++xb, bit = kMsbFirst ? bit - 1 : bit + 1)
+                        if (d[(wy * YB + yb) * X + wx * XB + xb] > 0) w |= static_cast<word_type>(word_type{1} << bit);
+                words[wy * (X / XB) + wx] = w;
+            }
         return words;
     }
 
@@ -78,19 +100,27 @@ struct RowWords {
     static infrastructure::Grid<2, cell_t> unpack(const std::vector<word_type>& words, std::size_t X, std::size_t Y) {
         infrastructure::Grid<2, cell_t> grid(X, Y);
         cell_t* d = grid.data();
-        for (std::size_t y = 0; y < Y; ++y)
-            for (std::size_t x = 0; x < X; ++x)
-                d[y * X + x] = static_cast<cell_t>((words[y * (X / Bits) + x / Bits] >> (x % Bits)) & 1);
+        for (std::size_t wy = 0; wy < Y / YB; ++wy)
+            for (std::size_t wx = 0; wx < X / XB; ++wx) {
+                const word_type w = words[wy * (X / XB) + wx];
+                std::size_t bit = kMsbFirst ? Bits - 1 : 0;
+                for (std::size_t yb = 0; yb < YB; ++yb)
+                    for (std::size_t xb = 0; xb < XB; ++xb, bit = kMsbFirst ? bit - 1 : bit + 1)
+                        d[(wy * YB + yb) * X + wx * XB + xb] = static_cast<cell_t>((w >> bit) & 1);
+            }
         return grid;
     }
 };
 
-template <typename grid_cell_t, std::size_t Bits, uint32_t Kernel>
+template <std::size_t Bits>
+using RowWords = LayoutWords<GOL_LAYOUT_ROWS, Bits>;
+
+template <typename grid_cell_t, std::size_t Bits, uint32_t Kernel, uint32_t Layout = GOL_LAYOUT_ROWS>
 class GoLHip : public infrastructure::Algorithm<2, grid_cell_t> {
    public:
     using size_type = std::size_t;
     using DataGrid = infrastructure::Grid<2, grid_cell_t>;
-    using Words = RowWords<Bits>;
+    using Words = LayoutWords<Layout, Bits>;
 
     GoLHip() = default;
     ~GoLHip() override { release(); }
@@ -108,6 +138,7 @@ class GoLHip : public infrastructure::Algorithm<2, grid_cell_t> {
         c.y_cells = y_;
         c.word_bits = static_cast<uint32_t>(Bits);
         c.kernel = Kernel;
+        c.layout = Layout;
         c.temporal_steps = this->params.temporal_steps > 1 ? static_cast<uint32_t>(this->params.temporal_steps) : 0;
         c.waves = this->params.thread_block_size >= 64 ? static_cast<uint32_t>(this->params.thread_block_size / 64) : 0;
         c.device = -1;
@@ -168,6 +199,15 @@ void register_hip_algorithms(infrastructure::AlgorithmRepository<2, cell_t>& rep
     repo.template register_algorithm<GoLHip<cell_t, 32, GOL_KERNEL_ACTIVE>>("gol-hip-local-32");
     repo.template register_algorithm<GoLHip<cell_t, 64, GOL_KERNEL_ACTIVE>>("gol-hip-local-64");
     repo.template register_algorithm<GoLHip<cell_t, 64, GOL_KERNEL_AUTO>>("gol-hip-auto-64");
+    // the reference's column and 8x8-tile layouts at the boundary (its cols / tiles
+    // algorithm families, experiment_manager.hpp:88-94, 134-140); the engine converts
+    // them to rows on the device
+    repo.template register_algorithm<GoLHip<cell_t, 16, GOL_KERNEL_AUTO, GOL_LAYOUT_COLUMNS>>("gol-hip-cols-16");
+    repo.template register_algorithm<GoLHip<cell_t, 32, GOL_KERNEL_AUTO, GOL_LAYOUT_COLUMNS>>("gol-hip-cols-32");
+    repo.template register_algorithm<GoLHip<cell_t, 64, GOL_KERNEL_AUTO, GOL_LAYOUT_COLUMNS>>("gol-hip-cols-64");
+    repo.template register_algorithm<GoLHip<cell_t, 16, GOL_KERNEL_AUTO, GOL_LAYOUT_TILES>>("gol-hip-tiles-16");
+    repo.template register_algorithm<GoLHip<cell_t, 32, GOL_KERNEL_AUTO, GOL_LAYOUT_TILES>>("gol-hip-tiles-32");
+    repo.template register_algorithm<GoLHip<cell_t, 64, GOL_KERNEL_AUTO, GOL_LAYOUT_TILES>>("gol-hip-tiles-64");
 }
 
 }  // namespace algorithms

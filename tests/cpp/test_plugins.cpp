@@ -21,6 +21,7 @@
 extern "C" {
 void oracle_load_random(uint8_t* cells, uint64_t x, uint64_t y, uint64_t seed);
 void oracle_naive_run(uint8_t* cells, uint64_t x, uint64_t y, uint64_t generations);
+int oracle_pack_layout(const uint8_t* cells, uint64_t x, uint64_t y, uint32_t layout, uint32_t word_bits, void* words);
 }
 
 static int failures = 0;
@@ -110,6 +111,31 @@ static void cpu_tests() {
     for (auto v : c) pop30 += v;
     CHECK(pop30 == 36 + 5 + 3);
 
+    // the plug-ins' packers against the oracle's, for every layout and word width
+    {
+        Grid g = infrastructure::RandomOnesZerosDataLoader<2, char>().load_data(params_for(128, 64, 0));
+        const std::vector<uint8_t> cells = oracle_cells(g);
+        auto check_layout = [&](auto words, uint32_t layout, uint32_t bits) {
+            using W = typename decltype(words)::value_type;
+            std::vector<W> want(words.size());
+            CHECK(oracle_pack_layout(cells.data(), 128, 64, layout, bits, want.data()) == 0);
+            CHECK(words == want);
+        };
+        check_layout(algorithms::LayoutWords<GOL_LAYOUT_ROWS, 32>::pack(g), GOL_LAYOUT_ROWS, 32);
+        check_layout(algorithms::LayoutWords<GOL_LAYOUT_ROWS, 64>::pack(g), GOL_LAYOUT_ROWS, 64);
+        check_layout(algorithms::LayoutWords<GOL_LAYOUT_COLUMNS, 16>::pack(g), GOL_LAYOUT_COLUMNS, 16);
+        check_layout(algorithms::LayoutWords<GOL_LAYOUT_COLUMNS, 32>::pack(g), GOL_LAYOUT_COLUMNS, 32);
+        check_layout(algorithms::LayoutWords<GOL_LAYOUT_COLUMNS, 64>::pack(g), GOL_LAYOUT_COLUMNS, 64);
+        check_layout(algorithms::LayoutWords<GOL_LAYOUT_TILES, 16>::pack(g), GOL_LAYOUT_TILES, 16);
+        check_layout(algorithms::LayoutWords<GOL_LAYOUT_TILES, 32>::pack(g), GOL_LAYOUT_TILES, 32);
+        check_layout(algorithms::LayoutWords<GOL_LAYOUT_TILES, 64>::pack(g), GOL_LAYOUT_TILES, 64);
+        using T64 = algorithms::LayoutWords<GOL_LAYOUT_TILES, 64>;
+        Grid back = T64::unpack<char>(T64::pack(g), 128, 64);
+        CHECK(oracle_cells(back) == cells);
+        Grid odd = infrastructure::ZerosLoader<2, char>().load_data(params_for(128, 20, 0));
+        CHECK(throws<std::invalid_argument>([&] { algorithms::LayoutWords<GOL_LAYOUT_COLUMNS, 16>::pack(odd); }));
+    }
+
     // always-changing: blinkers every 4 cells
     Grid ac = infrastructure::AlwaysChangingSpaceLoader<2, char>().load_data(params_for(8, 8, 0));
     CHECK(ac[0][0] == 1 && ac[2][0] == 1 && ac[3][0] == 0 && ac[4][4] == 1 && ac[0][1] == 0);
@@ -155,6 +181,34 @@ static void gpu_tests_for_encoding() {
         const std::size_t bits = name.find("-32") != std::string::npos ? 32 : 64;
         auto bad = params_for(bits + 32 == 64 ? 48 : 96, 8, 1);
         auto g = infrastructure::ZerosLoader<2, cell_t>().load_data(bad);
+        CHECK(throws<std::invalid_argument>([&] { alg->set_and_format_input_data(g); }));
+    }
+    // the column and tile layouts: heights a multiple of 64 (a 64-bit column word)
+    for (const std::string name : {"gol-hip-cols-16", "gol-hip-cols-32", "gol-hip-cols-64", "gol-hip-tiles-16",
+                                   "gol-hip-tiles-32", "gol-hip-tiles-64"}) {
+        std::cout << "[case] plug-in " << name << " cell bytes " << sizeof(cell_t) << std::endl;
+        auto* alg = repo.fetch_algorithm(name);
+        CHECK(alg != nullptr);
+        for (auto dims : {std::pair<std::size_t, std::size_t>{256, 128}, {128, 64}, {2048 + 128, 192}}) {
+            auto p = params_for(dims.first, dims.second, 45);
+            p.random_seed = dims.first + 3 * dims.second;
+            auto data = infrastructure::RandomOnesZerosDataLoader<2, cell_t>().load_data(p);
+            std::vector<uint8_t> ref(data.size());
+            for (std::size_t i = 0; i < data.size(); ++i) ref[i] = data.data()[i] > 0;
+            oracle_naive_run(ref.data(), dims.first, dims.second, p.iterations);
+            infrastructure::TimedAlgorithm<2, cell_t> timed(alg);
+            timed.set_params(p);
+            timed.set_and_format_input_data(data);
+            timed.initialize_data_structures();
+            timed.run(p.iterations);
+            timed.finalize_data_structures();
+            auto out = timed.fetch_result();
+            bool same = out.size() == ref.size();
+            for (std::size_t i = 0; same && i < ref.size(); ++i) same = (out.data()[i] != 0) == (ref[i] != 0);
+            CHECK(same);
+        }
+        // a height off the layout's word grid
+        auto g = infrastructure::ZerosLoader<2, cell_t>().load_data(params_for(128, 6, 1));
         CHECK(throws<std::invalid_argument>([&] { alg->set_and_format_input_data(g); }));
     }
     // temporal_steps and a time budget

@@ -35,6 +35,8 @@ def lib() -> ctypes.CDLL:
         L.oracle_naive_run_boundary.argtypes = [vp, u64, u64, u64, ctypes.c_int]
         L.oracle_pack_rows.argtypes = [vp, u64, u64, ctypes.c_uint32, vp]
         L.oracle_unpack_rows.argtypes = [vp, u64, u64, ctypes.c_uint32, vp]
+        L.oracle_pack_layout.argtypes = [vp, u64, u64, ctypes.c_uint32, ctypes.c_uint32, vp]
+        L.oracle_unpack_layout.argtypes = [vp, u64, u64, ctypes.c_uint32, ctypes.c_uint32, vp]
         L.oracle_bitpar_run.argtypes = [vp, u64, u64, u64, ctypes.c_int]
         L.oracle_popcount_words.argtypes = [vp, u64]
         L.oracle_popcount_words.restype = ctypes.c_uint64
@@ -84,6 +86,30 @@ def unpack(words: np.ndarray, x: int, y: int, word_bits: int = 64) -> np.ndarray
     rc = lib().oracle_unpack_rows(np.ascontiguousarray(words).ctypes.data, x, y, word_bits, c.ctypes.data)
     if rc:
         raise ValueError("X must be a multiple of word_bits")
+    return c.reshape(y, x)
+
+
+LAYOUT_ROWS, LAYOUT_COLUMNS, LAYOUT_TILES = 0, 1, 2
+_WORD_DTYPE = {16: np.uint16, 32: np.uint32, 64: np.uint64}
+
+
+def pack_layout(cells: np.ndarray, layout: int, word_bits: int) -> np.ndarray:
+    """The reference's GeneralBitGrid packing under the rows / columns / tiles
+    policy (general_bit_grid.hpp:217-246, bit_modes.hpp:10-163)."""
+    y, x = cells.shape
+    w = np.empty(x * y // word_bits, _WORD_DTYPE[word_bits])
+    rc = lib().oracle_pack_layout(np.ascontiguousarray(cells, dtype=np.uint8).ctypes.data, x, y, layout, word_bits,
+                                  w.ctypes.data)
+    if rc:
+        raise ValueError("grid size does not fit the layout's words")
+    return w
+
+
+def unpack_layout(words: np.ndarray, x: int, y: int, layout: int, word_bits: int) -> np.ndarray:
+    c = np.empty(x * y, np.uint8)
+    rc = lib().oracle_unpack_layout(np.ascontiguousarray(words).ctypes.data, x, y, layout, word_bits, c.ctypes.data)
+    if rc:
+        raise ValueError("grid size does not fit the layout's words")
     return c.reshape(y, x)
 
 

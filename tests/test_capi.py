@@ -57,6 +57,21 @@ def test_autotune_errors_before_device(size, rounds):
     assert e.value.code == golhip.GOL_ERR_INVALID_ARGUMENT
 
 
+@pytest.mark.parametrize("x,y,w,layout,multiple", [
+    (128, 40, 64, golhip.LAYOUT_COLUMNS, "Y must be a multiple of 64"),
+    (128, 12, 64, golhip.LAYOUT_TILES, "Y must be a multiple of 8"),
+    (128, 6, 32, golhip.LAYOUT_TILES, "Y must be a multiple of 4"),
+    (48, 64, 16, golhip.LAYOUT_COLUMNS, "X must be a multiple of 32"),
+    (64, 64, 16, golhip.LAYOUT_ROWS, "word_bits must be 32 or 64"),
+    (64, 64, 8, golhip.LAYOUT_TILES, "word_bits must be 16, 32 or 64"),
+])
+def test_layout_size_errors_before_device(x, y, w, layout, multiple):
+    with pytest.raises(golhip.GolError) as e:
+        golhip.Grid(x, y, word_bits=w, layout=layout)
+    assert e.value.code == golhip.GOL_ERR_INVALID_ARGUMENT
+    assert multiple in str(e.value)
+
+
 def test_error_message_names_the_multiple():
     with pytest.raises(golhip.GolError) as e:
         golhip.Grid(96, 8, word_bits=64)

@@ -167,6 +167,56 @@ def test_pack_places_bits():
     assert np.array_equal(ol.unpack(w32, 128, 3, 32), g)
 
 
+def expected_word_bit(layout, w, x_size, x, y):
+    """The reference's closed-form masks (bit_modes.hpp): FullyPackedRows bit x % W;
+    BitColumns bit y % W (X_BITS 1, Y_BITS W); BitTile MSB >> (y % Yb * Xb + x % Xb)
+    with Xb x Yb = 4x4 / 8x4 / 8x8; word (wx, wy) at wy * (X / X_BITS) + wx."""
+    if layout == ol.LAYOUT_ROWS:
+        xb, yb, bit = w, 1, x % w
+    elif layout == ol.LAYOUT_COLUMNS:
+        xb, yb, bit = 1, w, y % w
+    else:
+        xb, yb = (4, 4) if w == 16 else (8, 4) if w == 32 else (8, 8)
+        bit = w - 1 - ((y % yb) * xb + x % xb)
+    return (y // yb) * (x_size // xb) + x // xb, bit
+
+
+@pytest.mark.parametrize("layout", [ol.LAYOUT_ROWS, ol.LAYOUT_COLUMNS, ol.LAYOUT_TILES])
+@pytest.mark.parametrize("w", [16, 32, 64])
+def test_pack_layout_places_each_cell_by_the_closed_form_mask(layout, w):
+    # the oracle packs with the reference's loop (first mask, move to the next
+    # mask per cell in y_bit, x_bit order); the closed form is computed apart
+    x_size, y_size = 128, 128
+    rng = np.random.default_rng(layout * 100 + w)
+    for _ in range(40):
+        x, y = int(rng.integers(0, x_size)), int(rng.integers(0, y_size))
+        c = np.zeros((y_size, x_size), np.uint8)
+        c[y, x] = 1
+        words = ol.pack_layout(c, layout, w)
+        nz = np.flatnonzero(words)
+        idx, bit = expected_word_bit(layout, w, x_size, x, y)
+        assert list(nz) == [idx], (layout, w, x, y)
+        assert int(words[idx]) == 1 << bit, (layout, w, x, y)
+        assert np.array_equal(ol.unpack_layout(words, x_size, y_size, layout, w), c)
+
+
+@pytest.mark.parametrize("layout,w,x,y", [(ol.LAYOUT_COLUMNS, 16, 96, 48), (ol.LAYOUT_COLUMNS, 64, 40, 128),
+                                          (ol.LAYOUT_TILES, 16, 36, 20), (ol.LAYOUT_TILES, 32, 64, 12),
+                                          (ol.LAYOUT_TILES, 64, 24, 16), (ol.LAYOUT_ROWS, 32, 96, 7)])
+def test_pack_layout_round_trips(layout, w, x, y):
+    c = ol.bernoulli_cells(x, y, 5, 0.5)
+    assert np.array_equal(ol.unpack_layout(ol.pack_layout(c, layout, w), x, y, layout, w), c)
+    if layout == ol.LAYOUT_ROWS:
+        assert np.array_equal(ol.pack_layout(c, layout, w), ol.pack(c, w))
+
+
+def test_pack_layout_rejects_sizes_off_the_word_grid():
+    with pytest.raises(ValueError):
+        ol.pack_layout(np.zeros((20, 64), np.uint8), ol.LAYOUT_COLUMNS, 32)
+    with pytest.raises(ValueError):
+        ol.pack_layout(np.zeros((8, 12), np.uint8), ol.LAYOUT_TILES, 64)
+
+
 def test_pack_rejects_ragged_width():
     with pytest.raises(ValueError):
         ol.pack(np.zeros((2, 96), np.uint8), 64)
