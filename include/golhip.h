@@ -98,6 +98,11 @@ typedef struct gol_config {
 /* gol_config.flags */
 #define GOL_FLAG_NO_PERSISTENT 1u /* temporal kernel: one launch per block of generations, never the
                                      persistent slab launch */
+#define GOL_FLAG_SLAB_WAIT_PROBE 2u /* testing the failure path: in the persistent launch every slab
+                                       waits, with a short spin limit, for a block count no slab
+                                       publishes, so each wait ends by its time-out and the launch
+                                       stops; the next gol_sync / gol_download reports
+                                       GOL_ERR_RUNTIME and the handle is usable after an upload */
 
 typedef struct gol_info {
     uint32_t kernel;          /* resolved enum gol_kernel */

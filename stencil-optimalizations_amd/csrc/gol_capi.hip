@@ -267,9 +267,15 @@ int check_slab_error(gol_handle* h) {
     if (e != hipSuccess) return fail_hip(e, "slab error flag");
     h->slab_check = false;
     if (err) {
+        // the block counters are left at whatever the stopped launch reached:
+        // start them over, so the next launch synchronises from zero
         (void)hipMemsetAsync(h->slab_error, 0, sizeof(unsigned int), h->stream);
+        (void)hipMemsetAsync(h->slab_flags, 0, h->geom.tiles_y * sizeof(unsigned long long), h->stream);
+        (void)hipStreamSynchronize(h->stream);
+        h->slab_epoch = 0;
         return fail(GOL_ERR_RUNTIME, "persistent launch stopped: a slab waited too long for its neighbours; "
-                    "the state is undefined (set GOL_FLAG_NO_PERSISTENT to avoid the persistent launch)");
+                    "the state is undefined until the next upload (set GOL_FLAG_NO_PERSISTENT to avoid the "
+                    "persistent launch)");
     }
     return GOL_OK;
 }
@@ -294,6 +300,13 @@ int enqueue_persistent(gol_handle* h, uint64_t generations) {
         a.flags = h->slab_flags;
         a.epoch = h->slab_epoch;
         a.error = h->slab_error;
+        a.spin_limit = 0;
+        a.wait_ahead = 0;
+        if (h->cfg.flags & GOL_FLAG_SLAB_WAIT_PROBE) {
+            // every slab waits, briefly, for a block count no slab publishes
+            a.wait_ahead = 1ull << 40;
+            a.spin_limit = 256;
+        }
         hipError_t e = launch_slab(h->shape, a, h->stream);
         if (e != hipSuccess) return fail_hip(e, "slab_kernel launch");
         const uint64_t total = blocks + (last ? 1 : 0);

@@ -455,13 +455,14 @@ __global__ __launch_bounds__(64 * NW) void item_kernel(const BatchGrid* __restri
 // bounded: a slab that waits too long raises *error and every slab stops, so
 // the launch always ends.
 // ---------------------------------------------------------------------------
-constexpr int kSlabSpinLimit = 1 << 20;
+constexpr int kSlabSpinLimit = 1 << 20;  // polls before a wait gives up (SlabArgs::spin_limit 0)
 // Deepest halo (generations per block) a slab launch takes.
 constexpr int kSlabMaxSteps = 8;
 
 // Relaxed polling of a neighbour's block counter.
-__device__ bool wait_counter(const unsigned long long* p, unsigned long long target, unsigned int* error) {
-    for (int i = 0; i < kSlabSpinLimit; ++i) {
+__device__ bool wait_counter(const unsigned long long* p, unsigned long long target, unsigned int* error,
+                             int spin_limit) {
+    for (int i = 0; i < spin_limit; ++i) {
         if (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
         if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return false;
         __builtin_amdgcn_s_sleep(2);
@@ -532,8 +533,10 @@ __device__ __forceinline__ bool slab_blocks(const SlabArgs& a, uint32_t (&s)[BR]
             const unsigned long long target = a.epoch + static_cast<unsigned long long>(b) + 1;
             __hip_atomic_store(&a.flags[ty], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             bool ok = true;
-            if (ty > 0) ok = wait_counter(&a.flags[ty - 1], target, a.error);
-            if (ok && ty + 1 < a.tiles_y) ok = wait_counter(&a.flags[ty + 1], target, a.error);
+            const int spins = a.spin_limit > 0 ? a.spin_limit : kSlabSpinLimit;
+            const unsigned long long wanted = target + a.wait_ahead;
+            if (ty > 0) ok = wait_counter(&a.flags[ty - 1], wanted, a.error, spins);
+            if (ok && ty + 1 < a.tiles_y) ok = wait_counter(&a.flags[ty + 1], wanted, a.error, spins);
             *keep_going = ok ? 1 : 0;
         }
         __syncthreads();

@@ -118,6 +118,8 @@ struct SlabArgs {
     unsigned long long* flags;  // [tiles_y] blocks published so far, offset by epoch
     unsigned long long epoch;   // flag value every slab holds before this launch
     unsigned int* error;        // raised when a neighbour wait runs out
+    int spin_limit;             // polls before a wait gives up; 0: the built-in limit
+    unsigned long long wait_ahead;  // 0; testing the failure path: waits for counts no slab publishes
 };
 
 // Workgroups of the slab kernel that can be resident at once (false if the

@@ -73,6 +73,7 @@ class GolConfig(ctypes.Structure):
 
 
 FLAG_NO_PERSISTENT = 1
+FLAG_SLAB_WAIT_PROBE = 2
 
 LAYOUT_ROWS, LAYOUT_COLUMNS, LAYOUT_TILES = 0, 1, 2
 

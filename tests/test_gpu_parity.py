@@ -217,6 +217,29 @@ def test_persistent_repeated_launches_match_fixture():
             assert ol.sha256(g.download()) == f["sha256"], f"launch {i}"
 
 
+def test_persistent_wait_timeout_is_reported():
+    """The failure path of the persistent launch, forced from the host: every
+    slab waits, with a short spin limit, for a block count no slab publishes,
+    so each wait ends by its time-out and every slab leaves; the error reaches
+    the caller at the next synchronising call. A normal handle afterwards
+    matches the fixture (the device is unharmed)."""
+    w = golhip.fill_random(16384, 16384, 42)
+    f = [r for r in FIX["bitpar_runs"] if r["name"] == "random-16384" and r["generations"] == 1000][0]
+    with golhip.Grid(16384, 16384, flags=golhip.FLAG_SLAB_WAIT_PROBE) as g:
+        assert g.info().persistent == 1
+        g.upload(w)
+        note("slab wait probe")
+        g.step(1000)
+        with pytest.raises(golhip.GolError) as e:
+            g.sync()
+        assert e.value.code == golhip.GOL_ERR_RUNTIME and "waited too long" in str(e.value)
+        g.sync()  # reported once
+    with golhip.Grid(16384, 16384) as g:
+        g.upload(w)
+        g.step(1000)
+        assert ol.sha256(g.download()) == f["sha256"]
+
+
 def test_persistent_used_for_16384_and_split_calls():
     w = golhip.fill_random(16384, 16384, 42)
     f = [r for r in FIX["bitpar_runs"] if r["name"] == "random-16384" and r["generations"] == 1000][0]
