@@ -206,8 +206,9 @@ void gol_batch_destroy(gol_batch* b);
  * representative grid size it times every compiled tile shape (dwords per
  * lane 1/2/4/8, i.e. 32- to 256-bit lane words; rows per wavefront;
  * wavefronts per workgroup) and, for grids split into tiles, R, 3R/4, R/2,
- * 3R/8, R/4 and R/8 generations per round, on a batch of synthetic grids of
- * that size, and
+ * 3R/8, R/4 and R/8 generations per round, plus, for grids at most 512 cells
+ * wide and 512 rows tall, the narrow kernel (one grid row per lane), on a
+ * batch of synthetic grids of that size, and
  * installs the fastest for the size class (power-of-two classes of the row
  * width and the row count). The choices apply to every batch planned
  * afterwards in the process, until gol_batch_tuning_reset. Servers keep the
@@ -222,7 +223,8 @@ typedef struct gol_tune_config {
 
 typedef struct gol_tune_entry {
     uint64_t x_cells, y_cells;          /* in: a representative grid of the size class */
-    uint32_t lane_dwords, band_rows, waves;  /* out: the chosen tile shape */
+    uint32_t lane_dwords, band_rows, waves;  /* out: the chosen tile shape; band_rows 0 = the narrow
+                                                kernel (lane_dwords dwords per row, waves x 64 rows) */
     uint32_t generations_per_round;     /* out: for split grids; 0 when one tile spans the grid */
     uint32_t candidates;                /* out: choices timed */
     uint32_t grids_measured;            /* out: grids per measured batch */

@@ -12,10 +12,15 @@
 //     grid's buffers between rounds;
 //   * grids that fit one tile need no halo; they advance a chunk sized so the
 //     item costs about as much as a large tile's item, which keeps the items
-//     of a round balanced.
-// A round is one launch per tile shape present, one workgroup per item, items
-// ordered largest first. All rounds are recorded once into a hipGraph and
-// replayed, so a whole batch is one graph launch.
+//     of a round balanced. Grids at most 512 cells wide and 512 rows tall run
+//     on the narrow kernel (one row per lane, the whole grid in registers).
+// A round is one launch per launch key present (a tile shape, or a row width
+// of narrow grids), one workgroup per item, items ordered largest first. The
+// launch keys are spread over four streams (lanes) by total work, largest
+// first onto the least loaded lane; a key's launches stay on its lane, so the
+// lanes run their rounds independently between one fork and one join, and a
+// small shape fills the CUs a large one leaves idle. All rounds are recorded
+// once into a hipGraph and replayed, so a whole batch is one graph launch.
 
 #include <hip/hip_runtime.h>
 
@@ -36,8 +41,13 @@ namespace golhip {
 // Row budget of a round's work item, in tile rows x generations: a split
 // grid's item is (tile rows) x round_generations; one-tile grids get as many
 // generations per round as make the same product.
+// A narrow item's wavefronts each advance their rows' lane_dwords dwords plus
+// about two dwords' worth of edge exchange per generation, in parallel; it is
+// charged as a tile of 16 rows per dword of that sequential work, so its
+// latency stays near a large tile's item (one grid of few wavefronts must not
+// hold up a round).
 static int chunk_generations(const TileShape& s, int round_generations, int big_tile_rows) {
-    const int rows = s.band_rows * s.waves;
+    const int rows = is_narrow(s) ? 16 * (s.lane_dwords + 2) : s.band_rows * s.waves;
     const int64_t budget = static_cast<int64_t>(round_generations) * big_tile_rows;
     return static_cast<int>(std::max<int64_t>(round_generations, budget / rows));
 }
@@ -111,6 +121,15 @@ static bool plan_choice(const BatchChoice& c, int64_t x_dw, int64_t y_rows, int 
                         TileShape* shape, TileGeometry* geom, int* per_round) {
     const char* why = nullptr;
     const TileShape& s = c.shape;
+    if (is_narrow(s)) {
+        // a measured narrow choice means the narrow kernel, sized for this grid
+        TileShape n;
+        if (!narrow_shape_for(x_dw, y_rows, &n)) return false;
+        *shape = n;
+        *geom = narrow_geometry(n, x_dw, y_rows);
+        *per_round = chunk_generations(n, round_generations, big_tile_rows);
+        return true;
+    }
     if (!shape_compiled(s)) return false;
     if (x_dw <= 64 * s.lane_dwords && y_rows <= s.band_rows * s.waves) {
         if (!plan_tiles(s, x_dw, y_rows, 1, geom, &why)) return false;
@@ -140,6 +159,14 @@ int plan_batch_grid(int64_t x_dw, int64_t y_rows, int round_generations, bool us
     if (tuned_ok && tuned_choice(x_dw, y_rows, &tuned) &&
         plan_choice(tuned, x_dw, y_rows, round_generations, kBigTileRows, shape, geom, per_round))
         return GOL_OK;
+    // narrow grids: one row per lane, no halo
+    TileShape narrow;
+    if (narrow_shape_for(x_dw, y_rows, &narrow)) {
+        *shape = narrow;
+        *geom = narrow_geometry(narrow, x_dw, y_rows);
+        *per_round = chunk_generations(narrow, round_generations, kBigTileRows);
+        return GOL_OK;
+    }
     // grids that fit one tile of their size class need no halo
     const TileShape table = tuned_ok ? batch_shape(x_dw, y_rows) : batch_default_shape(x_dw, y_rows);
     if (x_dw <= 64 * table.lane_dwords && y_rows <= table.band_rows * table.waves) {
@@ -190,7 +217,11 @@ struct BatchEntry {
 struct gol_batch {
     gol_batch_config cfg{};
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // lane 0
+    static constexpr int kLanes = 4;
+    hipStream_t side[kLanes - 1] = {};  // lanes 1..3
+    hipEvent_t fork_ev = nullptr;
+    hipEvent_t join_ev[kLanes - 1] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<BatchEntry> grids;
     bool built = false;
@@ -202,6 +233,7 @@ struct gol_batch {
         TileShape shape;
         size_t first;  // index into the item array
         int count;
+        int lane;
     };
     std::vector<Launch> launches;
     uint32_t rounds = 0;
@@ -266,15 +298,40 @@ int build(gol_batch* b) {
     b->launches.clear();
     b->rounds = 0;
     b->cell_updates = 0;
+    // lanes: launch keys by total work, largest first onto the least loaded lane
+    std::map<LaunchKey, double> key_work;
+    for (const auto& g : b->grids)
+        key_work[launch_key(g.shape)] += static_cast<double>(g.x_dw * g.y_rows) * static_cast<double>(g.generations);
+    std::vector<std::pair<double, LaunchKey>> by_work;
+    for (const auto& kv : key_work) by_work.push_back({kv.second, kv.first});
+    std::stable_sort(by_work.begin(), by_work.end(),
+                     [](const std::pair<double, LaunchKey>& a, const std::pair<double, LaunchKey>& c) {
+                         return a.first > c.first;
+                     });
+    std::map<LaunchKey, int> lane_of;
+    double lane_work[gol_batch::kLanes] = {};
+    for (const auto& kw : by_work) {
+        int best = 0;
+        for (int l = 1; l < gol_batch::kLanes; ++l)
+            if (lane_work[l] < lane_work[best]) best = l;
+        lane_of[kw.second] = best;
+        lane_work[best] += kw.first;
+    }
+    struct Group {
+        std::vector<WorkItem> items;
+        int waves = 0;  // narrow grids: the tallest grid's wavefronts
+    };
     for (;;) {
-        std::map<std::tuple<int, int, int>, std::vector<WorkItem>> by_shape;
+        std::map<LaunchKey, Group> by_shape;
         bool any = false;
         for (size_t i = 0; i < b->grids.size(); ++i) {
             if (left[i] == 0) continue;
             any = true;
             auto& g = b->grids[i];
             const int gens = static_cast<int>(std::min<uint64_t>(left[i], static_cast<uint64_t>(g.chunk)));
-            auto& v = by_shape[{g.shape.lane_dwords, g.shape.band_rows, g.shape.waves}];
+            Group& grp = by_shape[launch_key(g.shape)];
+            grp.waves = std::max(grp.waves, g.shape.waves);
+            auto& v = grp.items;
             const int tiles = g.geom.tiles_x * g.geom.tiles_y;
             for (int t = 0; t < tiles; ++t) v.push_back(WorkItem{static_cast<int>(i), t, gens, parity[i]});
             parity[i] ^= 1;
@@ -283,12 +340,14 @@ int build(gol_batch* b) {
         }
         if (!any) break;
         for (auto& kv : by_shape) {
-            auto& v = kv.second;
+            auto& v = kv.second.items;
             std::stable_sort(v.begin(), v.end(), [](const WorkItem& a, const WorkItem& c) { return a.gens > c.gens; });
             gol_batch::Launch L;
-            L.shape = TileShape{std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first)};
+            const LaunchKey& key = kv.first;
+            L.shape = TileShape{key.lane_dwords, key.band_rows, key.band_rows ? key.waves : kv.second.waves};
             L.first = all.size();
             L.count = static_cast<int>(v.size());
+            L.lane = lane_of[key];
             all.insert(all.end(), v.begin(), v.end());
             b->launches.push_back(L);
         }
@@ -320,10 +379,28 @@ int build(gol_batch* b) {
     return GOL_OK;
 }
 
+// Fork the side lanes from the batch stream, run every lane's rounds, join.
 int enqueue_rounds(gol_batch* b) {
+    bool used[gol_batch::kLanes] = {};
+    for (const auto& L : b->launches) used[L.lane] = true;
+    hipError_t e;
+    bool forked = false;
+    for (int l = 1; l < gol_batch::kLanes; ++l) {
+        if (!used[l]) continue;
+        if (!forked && (e = hipEventRecord(b->fork_ev, b->stream)) != hipSuccess) return fail_hip(e, "batch fork");
+        forked = true;
+        if ((e = hipStreamWaitEvent(b->side[l - 1], b->fork_ev, 0)) != hipSuccess) return fail_hip(e, "batch fork");
+    }
     for (const auto& L : b->launches) {
-        hipError_t e = launch_items(L.shape, b->d_grids, b->d_items + L.first, L.count, b->stream);
-        if (e != hipSuccess) return fail_hip(e, "item_kernel launch");
+        hipStream_t st = L.lane == 0 ? b->stream : b->side[L.lane - 1];
+        if ((e = launch_items(L.shape, b->d_grids, b->d_items + L.first, L.count, st)) != hipSuccess)
+            return fail_hip(e, "item_kernel launch");
+    }
+    for (int l = 1; l < gol_batch::kLanes; ++l) {
+        if (!used[l]) continue;
+        if ((e = hipEventRecord(b->join_ev[l - 1], b->side[l - 1])) != hipSuccess ||
+            (e = hipStreamWaitEvent(b->stream, b->join_ev[l - 1], 0)) != hipSuccess)
+            return fail_hip(e, "batch join");
     }
     return GOL_OK;
 }
@@ -350,10 +427,19 @@ int gol_batch_create(const gol_batch_config* config, gol_batch** out) {
     b->device = dev;
     Guard g(dev);
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
-        (e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess) {
+        (e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming)) != hipSuccess) {
         int rc = fail_hip(e, "gol_batch_create");
         gol_batch_destroy(b);
         return rc;
+    }
+    for (int l = 0; l < gol_batch::kLanes - 1; ++l) {
+        if ((e = hipStreamCreateWithFlags(&b->side[l], hipStreamNonBlocking)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&b->join_ev[l], hipEventDisableTiming)) != hipSuccess) {
+            int rc = fail_hip(e, "gol_batch_create");
+            gol_batch_destroy(b);
+            return rc;
+        }
     }
     *out = b;
     return GOL_OK;
@@ -473,6 +559,8 @@ void gol_batch_destroy(gol_batch* b) {
     {
         Guard g(b->device);
         if (b->stream) (void)hipStreamSynchronize(b->stream);
+        for (hipStream_t st : b->side)
+            if (st) (void)hipStreamSynchronize(st);
         if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
         if (b->graph) (void)hipGraphDestroy(b->graph);
         if (b->arena) (void)hipFree(b->arena);
@@ -480,6 +568,11 @@ void gol_batch_destroy(gol_batch* b) {
         if (b->d_items) (void)hipFree(b->d_items);
         if (b->ev0) (void)hipEventDestroy(b->ev0);
         if (b->ev1) (void)hipEventDestroy(b->ev1);
+        if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
+        for (int l = 0; l < gol_batch::kLanes - 1; ++l) {
+            if (b->join_ev[l]) (void)hipEventDestroy(b->join_ev[l]);
+            if (b->side[l]) (void)hipStreamDestroy(b->side[l]);
+        }
         if (b->stream) (void)hipStreamDestroy(b->stream);
     }
     delete b;
@@ -558,6 +651,8 @@ int gol_batch_autotune(const gol_tune_config* config, gol_tune_entry* entries, u
         if (rc) return rc;
 
         std::vector<BatchChoice> cands;
+        TileShape narrow;
+        if (narrow_shape_for(x_dw, y_rows, &narrow)) cands.push_back(BatchChoice{narrow, 1});
         for (int i = 0; i < kNumTileShapes; ++i) {
             const TileShape& s = kTileShapes[i];
             if (x_dw <= 64 * s.lane_dwords && y_rows <= s.band_rows * s.waves) {
@@ -614,7 +709,8 @@ int gol_batch_autotune(const gol_tune_config* config, gol_tune_entry* entries, u
         out.lane_dwords = static_cast<uint32_t>(best.shape.lane_dwords);
         out.band_rows = static_cast<uint32_t>(best.shape.band_rows);
         out.waves = static_cast<uint32_t>(best.shape.waves);
-        const bool one_tile = x_dw <= 64 * best.shape.lane_dwords && y_rows <= best.shape.band_rows * best.shape.waves;
+        const bool one_tile = is_narrow(best.shape) ||
+                              (x_dw <= 64 * best.shape.lane_dwords && y_rows <= best.shape.band_rows * best.shape.waves);
         out.generations_per_round = one_tile ? 0u : static_cast<uint32_t>(best.steps);
         out.candidates = measured;
         out.grids_measured = static_cast<uint32_t>(count);

@@ -441,6 +441,93 @@ __global__ __launch_bounds__(64 * NW) void item_kernel(const BatchGrid* __restri
 }
 
 // ---------------------------------------------------------------------------
+// Narrow grids (batches and servers): lane l of wavefront w holds grid row
+// 64*w + l as DW dwords, the whole grid in registers for all the item's
+// generations, no halo. A row's west and east neighbours come from the lane's
+// own dwords, the rows above and below from the neighbouring lanes (DPP); the
+// rows across a wavefront edge come from the neighbouring wavefront's edge
+// lanes, which publish their 3-cell row sums in double-buffered LDS slots (one
+// barrier per generation). Rows and dwords outside the grid are cleared after
+// every generation (dead boundary). Narrow grids of one row width share a
+// launch sized for the tallest: wavefronts wholly below a grid only take part
+// in the barriers.
+// ---------------------------------------------------------------------------
+template <int DW>
+__global__ __launch_bounds__(64 * kNarrowMaxWaves) void narrow_item_kernel(const BatchGrid* __restrict__ grids,
+                                                                           const WorkItem* __restrict__ items) {
+    __shared__ uint32_t edge[2][kNarrowMaxWaves][4][DW];  // [parity][wave][top h0, top h1, bottom h0, bottom h1]
+    const WorkItem it = items[blockIdx.x];
+    const BatchGrid& bg = grids[it.grid];
+    const uint32_t* src = bg.buf[it.parity];
+    uint32_t* dst = bg.buf[it.parity ^ 1];
+    const int x_dw = bg.geo.x_dw, y_rows = bg.geo.y_rows;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));  // wave-uniform
+    const int waves_used = (y_rows + 63) >> 6;
+    if (w >= waves_used) {  // below the grid: keep the barrier count
+        for (int g = 0; g < it.gens; ++g) __syncthreads();
+        return;
+    }
+    const int y = w * 64 + lane;
+    const bool row_in = y < y_rows;
+    const uint32_t rowmask = row_in ? ~0u : 0u;
+    const uint32_t* row = src + static_cast<size_t>(row_in ? y : 0) * x_dw;
+    uint32_t s[DW];
+#pragma unroll
+    for (int j = 0; j < DW; ++j) s[j] = (j < x_dw ? row[j < x_dw ? j : 0] : 0u) & rowmask;
+    const bool mask_cols = x_dw < DW;  // wave-uniform
+    for (int g = 0; g < it.gens; ++g) {
+        uint32_t h0[DW], h1[DW], e0[DW], e1[DW];
+#pragma unroll
+        for (int j = 0; j < DW; ++j) {
+            const uint32_t wp = west_of(j > 0 ? s[j - 1] : 0u, s[j]);
+            const uint32_t ep = east_of(s[j], j + 1 < DW ? s[j + 1] : 0u);
+            const RowSum r = row_sum(wp, s[j], ep);
+            h0[j] = r.h0;
+            h1[j] = r.h1;
+            e0[j] = r.e0;
+            e1[j] = r.e1;
+        }
+        const int par = g & 1;
+        if (lane == 0) {
+#pragma unroll
+            for (int j = 0; j < DW; ++j) {
+                edge[par][w][0][j] = h0[j];
+                edge[par][w][1][j] = h1[j];
+            }
+        } else if (lane == 63) {
+#pragma unroll
+            for (int j = 0; j < DW; ++j) {
+                edge[par][w][2][j] = h0[j];
+                edge[par][w][3][j] = h1[j];
+            }
+        }
+        __syncthreads();
+        const int up = w > 0 ? w - 1 : 0, down = w + 1 < waves_used ? w + 1 : w;
+        const uint32_t up_ok = w > 0 ? ~0u : 0u, down_ok = w + 1 < waves_used ? ~0u : 0u;
+#pragma unroll
+        for (int j = 0; j < DW; ++j) {
+            // lane 0 takes the row above from the wavefront above, lane 63 the
+            // row below from the wavefront below (dead beyond the grid)
+            const uint32_t a0 = from_lower_lane_or(h0[j], edge[par][up][2][j] & up_ok);
+            const uint32_t a1 = from_lower_lane_or(h1[j], edge[par][up][3][j] & up_ok);
+            const uint32_t b0 = from_upper_lane_or(h0[j], edge[par][down][0][j] & down_ok);
+            const uint32_t b1 = from_upper_lane_or(h1[j], edge[par][down][1][j] & down_ok);
+            s[j] = next_state(a0, a1, e0[j], e1[j], b0, b1, s[j]) & rowmask;
+        }
+        if (mask_cols) {
+#pragma unroll
+            for (int j = 0; j < DW; ++j) s[j] = j < x_dw ? s[j] : 0u;
+        }
+    }
+    if (row_in) {
+#pragma unroll
+        for (int j = 0; j < DW; ++j)
+            if (j < x_dw) dst[static_cast<size_t>(y) * x_dw + j] = s[j];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Persistent slab kernel: one cooperative launch advances a grid whose rows a
 // single tile spans (tiles_x == 1) by many blocks of T generations. Workgroup
 // ty keeps its tile (interior rows plus T halo rows above and below) in VGPRs
@@ -759,9 +846,45 @@ int fine_resident_workgroups(int tile_dwords) {
     return cus * per_cu;
 }
 
+bool narrow_shape_for(int64_t x_dw, int64_t y_rows, TileShape* s) {
+    if (x_dw > kNarrowMaxDwords || y_rows > 64 * kNarrowMaxWaves) return false;
+    int dw = 2;
+    while (dw < x_dw) dw *= 2;
+    *s = TileShape{dw, 0, static_cast<int>((y_rows + 63) / 64)};
+    return true;
+}
+
+TileGeometry narrow_geometry(const TileShape& s, int64_t x_dw, int64_t y_rows) {
+    TileGeometry g{};
+    g.lane_dwords = s.lane_dwords;
+    g.band_rows = 0;
+    g.waves = s.waves;
+    g.tile_rows = 64 * s.waves;
+    g.tile_dwords = s.lane_dwords;
+    g.halo_rows = 0;
+    g.halo_lanes = 0;
+    g.stride_rows = static_cast<int>(y_rows);
+    g.stride_dwords = static_cast<int>(x_dw);
+    g.tiles_x = 1;
+    g.tiles_y = 1;
+    return g;
+}
+
 hipError_t launch_items(const TileShape& shape, const BatchGrid* grids, const WorkItem* items, int n_items,
                         hipStream_t stream) {
     if (n_items <= 0) return hipSuccess;
+    if (is_narrow(shape)) {
+        if (shape.waves < 1 || shape.waves > kNarrowMaxWaves) return hipErrorInvalidValue;
+        const dim3 grid(static_cast<unsigned>(n_items)), block(64 * shape.waves);
+        switch (shape.lane_dwords) {
+            case 2: hipLaunchKernelGGL(narrow_item_kernel<2>, grid, block, 0, stream, grids, items); break;
+            case 4: hipLaunchKernelGGL(narrow_item_kernel<4>, grid, block, 0, stream, grids, items); break;
+            case 8: hipLaunchKernelGGL(narrow_item_kernel<8>, grid, block, 0, stream, grids, items); break;
+            case 16: hipLaunchKernelGGL(narrow_item_kernel<16>, grid, block, 0, stream, grids, items); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
 #define GOLHIP_LAUNCH(DW, BR, NW)                                                                        \
     if (shape.lane_dwords == DW && shape.band_rows == BR && shape.waves == NW) {                         \
         hipLaunchKernelGGL((item_kernel<DW, BR, NW>), dim3(n_items), dim3(64 * NW), 0, stream, grids, items); \

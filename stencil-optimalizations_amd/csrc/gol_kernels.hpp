@@ -49,6 +49,32 @@ struct TileShape {
 extern const TileShape kTileShapes[];
 extern const int kNumTileShapes;
 
+// Narrow grids in batches and servers (at most kNarrowMaxDwords dwords wide and
+// 64 * kNarrowMaxWaves rows tall): lane l of wavefront w holds grid row
+// 64 * w + l whole, the grid stays in registers for the item's generations and
+// no halo is needed. A TileShape with band_rows == 0 names this kernel:
+// {dwords per row (2, 4, 8 or 16), 0, wavefronts (1..8)}.
+constexpr int kNarrowMaxDwords = 16;
+constexpr int kNarrowMaxWaves = 8;
+inline bool is_narrow(const TileShape& s) { return s.band_rows == 0; }
+// The launch a grid's items go to: one per tile shape, and one per row width
+// for narrow grids (launched with the tallest grid's wavefront count).
+struct LaunchKey {
+    int lane_dwords, band_rows, waves;
+    bool operator<(const LaunchKey& o) const {
+        if (lane_dwords != o.lane_dwords) return lane_dwords < o.lane_dwords;
+        if (band_rows != o.band_rows) return band_rows < o.band_rows;
+        return waves < o.waves;
+    }
+};
+inline LaunchKey launch_key(const TileShape& s) {
+    return is_narrow(s) ? LaunchKey{s.lane_dwords, 0, 0} : LaunchKey{s.lane_dwords, s.band_rows, s.waves};
+}
+// The narrow shape for a grid, if the grid is narrow enough.
+bool narrow_shape_for(int64_t x_dw, int64_t y_rows, TileShape* s);
+// Geometry of a narrow grid: one tile, no halo.
+TileGeometry narrow_geometry(const TileShape& s, int64_t x_dw, int64_t y_rows);
+
 // Compute the geometry for a grid of x_dw dwords x y_rows rows advanced by up
 // to max_gens generations per launch. Returns false (with a reason) when the
 // shape cannot cover max_gens generations.

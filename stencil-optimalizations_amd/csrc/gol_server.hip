@@ -1,20 +1,29 @@
 // Continuous-batching job server (no reference counterpart: the reference runs
 // one grid per process, src/infrastructure/experiment_manager.hpp:235-270).
 //
-// A scheduler thread owns one stream and runs rounds of work items, like a
+// A scheduler thread owns the streams and runs rounds of work items, like a
 // batch (gol_batch.hip): every resident job contributes one item per tile of
 // its grid, advancing it by its per-round generation count. Between rounds the
 // scheduler
 //   * admits queued jobs in arrival order while their packed state (two
 //     buffers) fits the HBM budget: a first-fit arena inside one device
 //     allocation, so admission never calls the device allocator;
-//   * uploads the new jobs' states and descriptors on the same stream, so they
-//     join the very next round: arrivals fold into the running batch;
+//   * uploads the new jobs' states and descriptors ahead of the next round,
+//     which waits for them: arrivals fold into the running batch;
 //   * copies finished jobs' results to pinned host memory after the round that
 //     finished them, and frees their arena space once that copy is done.
 // Planning runs ahead of the GPU (a job's progress is known on the host), and
 // up to two rounds are in flight. Each job's result is the same as running it
 // alone: items of different jobs never share buffers.
+// A round's launches (one per tile shape, one per row width of narrow grids)
+// touch disjoint jobs, so they run side by side on four compute lanes: a
+// stream each, with its own double-buffered item list. A launch key stays on
+// the lane it was first given (the least loaded) for the server's lifetime, so
+// each job's rounds are ordered on one stream and rounds need no cross-stream
+// waits; small shapes fill the CUs that a large shape's last workgroups leave
+// idle. Job transfers run on two copy streams beside the lanes: new jobs'
+// uploads on one, which their lane waits for, and result downloads on the
+// other, after the round that finished the job.
 
 #include <hip/hip_runtime.h>
 
@@ -55,6 +64,8 @@ struct ServerJob {
     TileGeometry geom{};
     int chunk = 0;
     int slot = -1;
+    int lane = -1;       // compute lane of the job's launch key
+    bool fresh = false;  // admitted in this pass: its lane waits for the upload
     size_t offset = 0, bytes = 0;  // arena block holding buf0 then buf1
     size_t dwords = 0;
     uint32_t* pinned_in = nullptr;
@@ -119,18 +130,27 @@ struct gol_server {
     gol_server_config cfg{};
     int device = 0;
     int round_generations = 32;
-    hipStream_t stream = nullptr;
     uint8_t* arena = nullptr;
     size_t arena_bytes = 0;
     ArenaAllocator alloc;
     BatchGrid* d_desc = nullptr;
     BatchGrid* h_desc = nullptr;  // pinned staging, one entry per slot
     std::vector<int> free_slots;
-    WorkItem* d_items[2] = {nullptr, nullptr};
-    WorkItem* h_items[2] = {nullptr, nullptr};
-    size_t items_cap = 0;
-    hipEvent_t round_ev[2] = {nullptr, nullptr};
-    bool round_ev_used[2] = {false, false};
+    struct Lane {
+        hipStream_t stream = nullptr;
+        WorkItem* d_items[2] = {nullptr, nullptr};
+        WorkItem* h_items[2] = {nullptr, nullptr};  // pinned staging
+        size_t cap = 0;
+        hipEvent_t round_ev[2] = {nullptr, nullptr};
+        bool used[2] = {false, false};
+        double work = 0;  // dword-generations given so far, to place new launch keys
+    };
+    static constexpr int kLanes = 4;
+    Lane lanes[kLanes];
+    std::map<LaunchKey, int> lane_of;
+    hipStream_t up = nullptr, down = nullptr;  // host-to-device and device-to-host copies
+    hipEvent_t up_ev = nullptr;                 // the pass's job uploads are done
+    hipEvent_t up_all_ev = nullptr;             // every upload so far is done (orders downloads)
     int next_buf = 0;
 
     std::mutex mu;
@@ -155,25 +175,36 @@ void free_job(ServerJob* j) {
     delete j;
 }
 
-int ensure_item_capacity(gol_server* s, size_t n) {
-    if (n <= s->items_cap) return GOL_OK;
-    hipError_t e = hipStreamSynchronize(s->stream);
-    if (e != hipSuccess) return fail_hip(e, "server stream");
-    size_t cap = std::max<size_t>(n, 2 * s->items_cap);
+int ensure_item_capacity(gol_server::Lane& L, size_t n) {
+    if (n <= L.cap) return GOL_OK;
+    hipError_t e;
+    if ((e = hipStreamSynchronize(L.stream)) != hipSuccess) return fail_hip(e, "server lane");
+    size_t cap = std::max<size_t>(n, 2 * L.cap);
     for (int k = 0; k < 2; ++k) {
-        if (s->d_items[k]) (void)hipFree(s->d_items[k]);
-        if (s->h_items[k]) (void)hipHostFree(s->h_items[k]);
-        s->d_items[k] = nullptr;
-        s->h_items[k] = nullptr;
-        if ((e = hipMalloc(&s->d_items[k], cap * sizeof(WorkItem))) != hipSuccess)
-            return fail_hip(e, "server items");
-        if ((e = hipHostMalloc(reinterpret_cast<void**>(&s->h_items[k]), cap * sizeof(WorkItem), hipHostMallocDefault)) !=
+        if (L.d_items[k]) (void)hipFree(L.d_items[k]);
+        if (L.h_items[k]) (void)hipHostFree(L.h_items[k]);
+        L.d_items[k] = nullptr;
+        L.h_items[k] = nullptr;
+        if ((e = hipMalloc(&L.d_items[k], cap * sizeof(WorkItem))) != hipSuccess) return fail_hip(e, "server items");
+        if ((e = hipHostMalloc(reinterpret_cast<void**>(&L.h_items[k]), cap * sizeof(WorkItem), hipHostMallocDefault)) !=
             hipSuccess)
             return fail_hip(e, "server items (host)");
     }
-    s->items_cap = cap;
-    s->round_ev_used[0] = s->round_ev_used[1] = false;
+    L.cap = cap;
+    L.used[0] = L.used[1] = false;
     return GOL_OK;
+}
+
+// The lane of a launch key: fixed once given, new keys to the least loaded lane.
+int lane_for(gol_server* s, const TileShape& sh) {
+    const LaunchKey key = launch_key(sh);
+    auto it = s->lane_of.find(key);
+    if (it != s->lane_of.end()) return it->second;
+    int best = 0;
+    for (int i = 1; i < gol_server::kLanes; ++i)
+        if (s->lanes[i].work < s->lanes[best].work) best = i;
+    s->lane_of[key] = best;
+    return best;
 }
 
 // One pass of the scheduler; returns false when the worker should exit.
@@ -203,6 +234,7 @@ bool schedule_once(gol_server* s) {
             j->slot = s->free_slots.back();
             s->free_slots.pop_back();
             j->state = kResident;
+            j->fresh = true;
             j->t_admit = Clock::now();
             admitted.push_back(j);
             s->stats.admitted += 1;
@@ -217,85 +249,128 @@ bool schedule_once(gol_server* s) {
         s->h_desc[j->slot].buf[0] = buf0;
         s->h_desc[j->slot].buf[1] = buf0 + stride;
         s->h_desc[j->slot].geo = to_device_geo(j->geom, j->x_dw, j->y_rows);
-        if ((e = hipMemcpyAsync(buf0, j->pinned_in, j->dwords * 4, hipMemcpyHostToDevice, s->stream)) != hipSuccess ||
+        if ((e = hipMemcpyAsync(buf0, j->pinned_in, j->dwords * 4, hipMemcpyHostToDevice, s->up)) != hipSuccess ||
             (e = hipMemcpyAsync(s->d_desc + j->slot, s->h_desc + j->slot, sizeof(BatchGrid), hipMemcpyHostToDevice,
-                                s->stream)) != hipSuccess) {
+                                s->up)) != hipSuccess) {
             s->worker_error = fail_hip(e, "server upload");
             return false;
         }
         s->resident.push_back(j);
     }
 
-    // one round over the resident jobs, grouped by tile shape
-    std::map<std::tuple<int, int, int>, std::vector<WorkItem>> by_shape;
+    // one round over the resident jobs: per lane, one launch per launch key
+    struct Group {
+        std::vector<WorkItem> items;
+        int waves = 0;  // narrow grids: the tallest grid's wavefronts
+    };
+    std::map<LaunchKey, Group> groups[gol_server::kLanes];
+    bool lane_waits_upload[gol_server::kLanes] = {};
+    bool lane_finishes[gol_server::kLanes] = {};
     size_t n_items = 0;
     std::vector<ServerJob*> finished;
     uint64_t round_updates = 0;
     for (ServerJob* j : s->resident) {
-        if (j->remaining == 0) {
+        if (j->remaining == 0) {  // no generations: never joins a round
             finished.push_back(j);
             continue;
         }
         const int gens = static_cast<int>(std::min<uint64_t>(j->remaining, static_cast<uint64_t>(j->chunk)));
-        auto& v = by_shape[{j->shape.lane_dwords, j->shape.band_rows, j->shape.waves}];
+        j->lane = lane_for(s, j->shape);
+        Group& g = groups[j->lane][launch_key(j->shape)];
+        g.waves = std::max(g.waves, j->shape.waves);
         const int tiles = j->geom.tiles_x * j->geom.tiles_y;
-        for (int t = 0; t < tiles; ++t) v.push_back(WorkItem{j->slot, t, gens, j->parity});
+        for (int t = 0; t < tiles; ++t) g.items.push_back(WorkItem{j->slot, t, gens, j->parity});
         n_items += static_cast<size_t>(tiles);
+        if (j->fresh) lane_waits_upload[j->lane] = true;
+        j->fresh = false;
         j->parity ^= 1;
         j->remaining -= static_cast<uint64_t>(gens);
         j->rounds += 1;
-        round_updates += static_cast<uint64_t>(j->x_dw * 32 * j->y_rows) * static_cast<uint64_t>(gens);
-        if (j->remaining == 0) finished.push_back(j);
+        const uint64_t updates = static_cast<uint64_t>(j->x_dw * 32 * j->y_rows) * static_cast<uint64_t>(gens);
+        round_updates += updates;
+        s->lanes[j->lane].work += static_cast<double>(updates);
+        if (j->remaining == 0) {
+            finished.push_back(j);
+            lane_finishes[j->lane] = true;
+        }
     }
+    const int k = s->next_buf;
     if (n_items > 0) {
-        int rc = ensure_item_capacity(s, n_items);
-        if (rc) {
-            s->worker_error = rc;
+        if (!admitted.empty() && (e = hipEventRecord(s->up_ev, s->up)) != hipSuccess) {
+            s->worker_error = fail_hip(e, "server upload order");
             return false;
         }
-        const int k = s->next_buf;
-        if (s->round_ev_used[k] && (e = hipEventSynchronize(s->round_ev[k])) != hipSuccess) {
-            s->worker_error = fail_hip(e, "server round wait");
-            return false;
-        }
-        size_t at = 0;
-        std::vector<std::pair<TileShape, std::pair<size_t, int>>> launches;
-        for (auto& kv : by_shape) {
-            auto& v = kv.second;
-            std::memcpy(s->h_items[k] + at, v.data(), v.size() * sizeof(WorkItem));
-            launches.push_back({TileShape{std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first)},
-                                {at, static_cast<int>(v.size())}});
-            at += v.size();
-        }
-        if ((e = hipMemcpyAsync(s->d_items[k], s->h_items[k], at * sizeof(WorkItem), hipMemcpyHostToDevice,
-                                s->stream)) != hipSuccess) {
-            s->worker_error = fail_hip(e, "server items upload");
-            return false;
-        }
-        for (auto& L : launches) {
-            if ((e = launch_items(L.first, s->d_desc, s->d_items[k] + L.second.first, L.second.second, s->stream)) !=
-                hipSuccess) {
-                s->worker_error = fail_hip(e, "server round launch");
+        for (int ln = 0; ln < gol_server::kLanes; ++ln) {
+            if (groups[ln].empty()) continue;
+            gol_server::Lane& L = s->lanes[ln];
+            size_t count = 0;
+            for (auto& kv : groups[ln]) count += kv.second.items.size();
+            // the lane's items buffer k was last read two passes ago
+            if (L.used[k] && (e = hipEventSynchronize(L.round_ev[k])) != hipSuccess) {
+                s->worker_error = fail_hip(e, "server round wait");
                 return false;
             }
+            int rc = ensure_item_capacity(L, count);
+            if (rc) {
+                s->worker_error = rc;
+                return false;
+            }
+            size_t at = 0;
+            for (auto& kv : groups[ln]) {
+                std::memcpy(L.h_items[k] + at, kv.second.items.data(), kv.second.items.size() * sizeof(WorkItem));
+                at += kv.second.items.size();
+            }
+            // new jobs' uploads, then the items (a few KB, on the lane itself)
+            if ((lane_waits_upload[ln] && (e = hipStreamWaitEvent(L.stream, s->up_ev, 0)) != hipSuccess) ||
+                (e = hipMemcpyAsync(L.d_items[k], L.h_items[k], at * sizeof(WorkItem), hipMemcpyHostToDevice,
+                                    L.stream)) != hipSuccess) {
+                s->worker_error = fail_hip(e, "server items upload");
+                return false;
+            }
+            at = 0;
+            for (auto& kv : groups[ln]) {
+                const LaunchKey& key = kv.first;
+                const TileShape sh{key.lane_dwords, key.band_rows, key.band_rows ? key.waves : kv.second.waves};
+                const int n = static_cast<int>(kv.second.items.size());
+                if ((e = launch_items(sh, s->d_desc, L.d_items[k] + at, n, L.stream)) != hipSuccess) {
+                    s->worker_error = fail_hip(e, "server round launch");
+                    return false;
+                }
+                at += static_cast<size_t>(n);
+            }
+            if ((e = hipEventRecord(L.round_ev[k], L.stream)) != hipSuccess) {
+                s->worker_error = fail_hip(e, "hipEventRecord");
+                return false;
+            }
+            L.used[k] = true;
         }
-        if ((e = hipEventRecord(s->round_ev[k], s->stream)) != hipSuccess) {
-            s->worker_error = fail_hip(e, "hipEventRecord");
-            return false;
-        }
-        s->round_ev_used[k] = true;
         s->next_buf ^= 1;
         std::lock_guard<std::mutex> lk(s->mu);
         s->stats.rounds += 1;
         s->stats.items += n_items;
         s->stats.cell_updates += round_updates;
     }
-    // results of the jobs this round finished, ordered after the round
+    // results of the jobs this round finished, ordered after their lanes'
+    // round and after every upload (a job with no generations never joins a
+    // round)
+    if (!finished.empty()) {
+        for (int ln = 0; ln < gol_server::kLanes; ++ln) {
+            if (lane_finishes[ln] && (e = hipStreamWaitEvent(s->down, s->lanes[ln].round_ev[k], 0)) != hipSuccess) {
+                s->worker_error = fail_hip(e, "server download order");
+                return false;
+            }
+        }
+        if ((e = hipEventRecord(s->up_all_ev, s->up)) != hipSuccess ||
+            (e = hipStreamWaitEvent(s->down, s->up_all_ev, 0)) != hipSuccess) {
+            s->worker_error = fail_hip(e, "server download order");
+            return false;
+        }
+    }
     for (ServerJob* j : finished) {
         const size_t stride = ((j->dwords + 63) & ~size_t(63));
         const uint32_t* src = reinterpret_cast<const uint32_t*>(s->arena + j->offset) + (j->parity ? stride : 0);
-        if ((e = hipMemcpyAsync(j->pinned_out, src, j->dwords * 4, hipMemcpyDeviceToHost, s->stream)) != hipSuccess ||
-            (e = hipEventRecord(j->done_ev, s->stream)) != hipSuccess) {
+        if ((e = hipMemcpyAsync(j->pinned_out, src, j->dwords * 4, hipMemcpyDeviceToHost, s->down)) != hipSuccess ||
+            (e = hipEventRecord(j->done_ev, s->down)) != hipSuccess) {
             s->worker_error = fail_hip(e, "server download");
             return false;
         }
@@ -332,11 +407,18 @@ bool schedule_once(gol_server* s) {
     return true;
 }
 
+void sync_streams(gol_server* s) {
+    for (hipStream_t st : {s->up, s->down})
+        if (st) (void)hipStreamSynchronize(st);
+    for (auto& L : s->lanes)
+        if (L.stream) (void)hipStreamSynchronize(L.stream);
+}
+
 void worker_main(gol_server* s) {
     (void)hipSetDevice(s->device);
     while (schedule_once(s)) {
     }
-    (void)hipStreamSynchronize(s->stream);
+    sync_streams(s);
     std::lock_guard<std::mutex> lk(s->mu);
     // anything left is failed (server stopping or a device error)
     const int code = s->worker_error ? s->worker_error : GOL_ERR_RUNTIME;
@@ -353,16 +435,22 @@ void worker_main(gol_server* s) {
 }
 
 void destroy_resources(gol_server* s) {
-    if (s->stream) (void)hipStreamSynchronize(s->stream);
-    for (int k = 0; k < 2; ++k) {
-        if (s->d_items[k]) (void)hipFree(s->d_items[k]);
-        if (s->h_items[k]) (void)hipHostFree(s->h_items[k]);
-        if (s->round_ev[k]) (void)hipEventDestroy(s->round_ev[k]);
+    sync_streams(s);
+    for (auto& L : s->lanes) {
+        for (int k = 0; k < 2; ++k) {
+            if (L.d_items[k]) (void)hipFree(L.d_items[k]);
+            if (L.h_items[k]) (void)hipHostFree(L.h_items[k]);
+            if (L.round_ev[k]) (void)hipEventDestroy(L.round_ev[k]);
+        }
+        if (L.stream) (void)hipStreamDestroy(L.stream);
     }
+    for (hipEvent_t ev : {s->up_ev, s->up_all_ev})
+        if (ev) (void)hipEventDestroy(ev);
+    for (hipStream_t st : {s->up, s->down})
+        if (st) (void)hipStreamDestroy(st);
     if (s->d_desc) (void)hipFree(s->d_desc);
     if (s->h_desc) (void)hipHostFree(s->h_desc);
     if (s->arena) (void)hipFree(s->arena);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
 }  // namespace
@@ -389,17 +477,28 @@ int gol_server_create(const gol_server_config* config, gol_server** out) {
     s->round_generations = cfg.round_generations ? static_cast<int>(cfg.round_generations) : 32;
     (void)hipSetDevice(dev);
     s->arena_bytes = (cfg.hbm_budget_bytes + 255) & ~uint64_t(255);
-    if ((e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess ||
-        (e = hipMalloc(&s->arena, s->arena_bytes)) != hipSuccess ||
+    if ((e = hipMalloc(&s->arena, s->arena_bytes)) != hipSuccess ||
         (e = hipMalloc(&s->d_desc, cfg.max_jobs * sizeof(BatchGrid))) != hipSuccess ||
         (e = hipHostMalloc(reinterpret_cast<void**>(&s->h_desc), cfg.max_jobs * sizeof(BatchGrid),
                            hipHostMallocDefault)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&s->round_ev[0], hipEventDisableTiming)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&s->round_ev[1], hipEventDisableTiming)) != hipSuccess) {
+        (e = hipEventCreateWithFlags(&s->up_ev, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&s->up_all_ev, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&s->down, hipStreamNonBlocking)) != hipSuccess) {
         int rc = fail_hip(e, "gol_server_create");
         destroy_resources(s);
         delete s;
         return rc;
+    }
+    for (auto& L : s->lanes) {
+        if ((e = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&L.round_ev[0], hipEventDisableTiming)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&L.round_ev[1], hipEventDisableTiming)) != hipSuccess) {
+            int rc = fail_hip(e, "gol_server_create");
+            destroy_resources(s);
+            delete s;
+            return rc;
+        }
     }
     s->alloc.reset(s->arena_bytes);
     for (int i = static_cast<int>(cfg.max_jobs) - 1; i >= 0; --i) s->free_slots.push_back(i);

@@ -75,6 +75,42 @@ def test_same_alone_and_in_the_mix():
                 assert np.array_equal(g.download(), b.fetch(i))
 
 
+@pytest.mark.parametrize("round_generations", [32, 5])
+def test_narrow_grids_in_batch(round_generations):
+    """Grids at most 512 cells wide and 512 rows tall run on the narrow kernel
+    (one grid row per lane, 2/4/8/16 dwords per lane, 1..8 wavefronts): every
+    width class with ragged widths, heights across wavefront edges, and long
+    runs that advance hundreds of generations per item."""
+    specs = []
+    for i, (x, y) in enumerate([(32, 1), (64, 2), (96, 63), (128, 64), (160, 65), (256, 127), (288, 128),
+                                (384, 200), (480, 255), (512, 256), (32, 512), (512, 512), (448, 449),
+                                (224, 320), (352, 383), (64, 448)]):
+        gens = [0, 1, 37, 500, 1500][i % 5]
+        specs.append((x, y, gens, ol.bernoulli_cells(x, y, 300 + i, 0.25 + 0.05 * (i % 5))))
+    with golhip.Batch(round_generations=round_generations) as b:
+        for x, y, gens, c in specs:
+            b.add(ol.pack(c, 32), x, y, gens, word_bits=32)
+        note("narrow grids, round_generations", round_generations)
+        b.run()
+        for i, (x, y, gens, c) in enumerate(specs):
+            assert np.array_equal(ol.unpack(b.fetch(i), x, y, 32), ol.naive_run(c, gens)), (x, y, gens)
+
+
+def test_narrow_oscillators_at_every_edge():
+    # blinkers across every wavefront edge and against the grid's edges: the
+    # rows exchanged between wavefronts and the dead boundary
+    x, y = 512, 512
+    c = np.zeros((y, x), dtype=np.uint8)
+    for r in list(range(0, 512, 64)) + list(range(63, 512, 64)) + [1, 510]:
+        for col in (0, 31, 32, 255, 256, 479, 510):
+            c[max(r - 1, 0):min(r + 2, y), col] = 1
+    with golhip.Batch() as b:
+        gids = [b.add(ol.pack(c, 32), x, y, g, word_bits=32) for g in (1, 2, 101)]
+        b.run()
+        for gid, g in zip(gids, (1, 2, 101)):
+            assert np.array_equal(ol.unpack(b.fetch(gid), x, y, 32), ol.naive_run(c, g)), g
+
+
 def test_word_bits_32_in_batch():
     with golhip.Batch() as b:
         c = ol.bernoulli_cells(96, 50, 3, 0.5)
@@ -94,9 +130,14 @@ def test_autotune_installs_measured_choices():
         for r in res:
             assert r["candidates"] >= 1
             assert r["tuned_cell_updates_per_s"] >= r["builtin_cell_updates_per_s"] > 0
-            assert (r["lane_dwords"], r["band_rows"], r["waves"]) in {tuple(s) for s in golhip.TILE_SHAPES}
-            one_tile = (r["x_cells"] <= 2048 * r["lane_dwords"]
-                        and r["y_cells"] <= r["band_rows"] * r["waves"])
+            narrow = r["band_rows"] == 0
+            if narrow:  # the narrow kernel: one grid row per lane
+                assert r["x_cells"] <= 512 and r["y_cells"] <= 512
+                assert r["x_cells"] <= 32 * r["lane_dwords"] and r["y_cells"] <= 64 * r["waves"]
+            else:
+                assert (r["lane_dwords"], r["band_rows"], r["waves"]) in {tuple(s) for s in golhip.TILE_SHAPES}
+            one_tile = narrow or (r["x_cells"] <= 2048 * r["lane_dwords"]
+                                  and r["y_cells"] <= r["band_rows"] * r["waves"])
             assert (r["generations_per_round"] == 0) == one_tile
             if not one_tile:
                 assert 1 <= r["generations_per_round"] <= 16
