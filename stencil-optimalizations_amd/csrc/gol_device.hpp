@@ -118,7 +118,10 @@ __device__ __forceinline__ uint32_t from_upper_lane_or(uint32_t v, uint32_t edge
 // rows x 64*DW dwords) by one generation, in place.
 //   above/below: the rows just outside the band (same dword columns).
 //   MASK_ROWS: band rows outside [row_lo, row_hi) (wave-uniform) lie outside
-//   the grid and are cleared after the update (dead boundary).
+//   the grid. Only the row next to the grid (row_lo - 1, row_hi) is cleared
+//   after the update: rows further out reach the grid only through it, so
+//   the grid sees a dead boundary while they hold whatever they evolve to.
+//   The clearing is a wave-uniform branch per row, no VALU work per dword.
 //   MASK_COLS: dwords with colmask 0 lie outside the grid and are cleared.
 //   EDGE_IN: wave_edge_l/r[i] hold the dword just left of lane 0's first dword
 //   / right of lane 63's last dword for rows above (i=0), band (1..BR) and
@@ -201,7 +204,6 @@ __device__ __forceinline__ void advance_band(uint32_t (&s)[BR][DW], const uint32
             if (d < DW - 1) wnext[r] = west_of(s[r][d], s[r][d + 1]);
             uint32_t v = next_state(pa0, pa1, cur.e0, cur.e1, nxt.h0, nxt.h1, s[r][d]);
             if (MASK_COLS) v &= colmask[d];
-            if (MASK_ROWS) v = (r >= row_lo && r < row_hi) ? v : 0u;
             s[r][d] = v;
             pa0 = cur.h0;
             pa1 = cur.h1;
@@ -209,6 +211,17 @@ __device__ __forceinline__ void advance_band(uint32_t (&s)[BR][DW], const uint32
             // keep the scheduler from hoisting later rows' cross-lane reads
             // (it would hold every edge of the band live at once)
             __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    if (MASK_ROWS) {
+        // within a generation only old values are read, so clearing the
+        // boundary row now equals clearing it as it is computed
+#pragma unroll
+        for (int r = 0; r < BR; ++r) {
+            if (r == row_lo - 1 || r == row_hi) {
+#pragma unroll
+                for (int d = 0; d < DW; ++d) s[r][d] = 0u;
+            }
         }
     }
 }

@@ -583,18 +583,20 @@ __device__ __forceinline__ unsigned long long fetch_pair(const unsigned long lon
 
 // The block loop of one slab; the row and column masks are fixed for the
 // slab, so each instantiation holds one copy of the band update.
-template <int DW, int BR, int NW, int STAGE_ROWS, bool MASK_ROWS, bool MASK_COLS>
+template <int DW, int BR, int NW, bool MASK_ROWS, bool MASK_COLS>
 __device__ __forceinline__ bool slab_blocks(const SlabArgs& a, uint32_t (&s)[BR][DW], EdgeRows<DW, NW>& lds,
-                                            uint32_t* halo_stage, int* keep_going, int lane, int w, int gx0,
-                                            int row_lo, int row_hi, const uint32_t (&colmask)[DW]) {
+                                            int* keep_going, int lane, int w, int row_lo, int row_hi,
+                                            const uint32_t (&colmask)[DW]) {
     constexpr int RP = 32 * DW;  // dword pairs per exchanged row (one wavefront's 64 segments)
-    constexpr int K = (kSlabMaxSteps * RP + 64 * NW - 1) / (64 * NW);  // pairs per thread and side
-    const TileGeo& g = a.geo;
     const int ty = blockIdx.x;
     const int T = a.steps;
     const int tile_rows = NW * BR;
     const size_t plane = static_cast<size_t>(T) * RP;  // pairs of one side: T rows
     const int total_blocks = a.blocks + (a.last_gens > 0 ? 1 : 0);
+    // wavefronts holding halo rows import them (wave-uniform)
+    const bool import_top = ty > 0 && w * BR < T;
+    const bool import_bottom = ty + 1 < a.tiles_y && (w + 1) * BR > tile_rows - T;
+    const int spins = a.spin_limit > 0 ? a.spin_limit : kSlabSpinLimit;
     for (int b = 0; b < total_blocks; ++b) {
         const int gens = b < a.blocks ? T : a.last_gens;
         run_generations<DW, BR, NW, MASK_ROWS, MASK_COLS>(s, lds, gens, lane, w, row_lo, row_hi, colmask);
@@ -612,93 +614,63 @@ __device__ __forceinline__ bool slab_blocks(const SlabArgs& a, uint32_t (&s)[BR]
         }
         // Publish: every wave waits for its device-scope stores to be
         // acknowledged (vmcnt 0) before the barrier; then one thread bumps the
-        // counter. Subscribe: the same thread polls both neighbours; after the
-        // barrier every wave reads their rows with device-scope loads.
+        // counter.
+        if (threadIdx.x == 0) *keep_going = 1;
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        if (threadIdx.x == 0) {
-            const unsigned long long target = a.epoch + static_cast<unsigned long long>(b) + 1;
-            __hip_atomic_store(&a.flags[ty], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            bool ok = true;
-            const int spins = a.spin_limit > 0 ? a.spin_limit : kSlabSpinLimit;
-            const unsigned long long wanted = target + a.wait_ahead;
-            if (ty > 0) ok = wait_counter(&a.flags[ty - 1], wanted, a.error, spins);
-            if (ok && ty + 1 < a.tiles_y) ok = wait_counter(&a.flags[ty + 1], wanted, a.error, spins);
-            *keep_going = ok ? 1 : 0;
+        const unsigned long long target = a.epoch + static_cast<unsigned long long>(b) + 1;
+        if (threadIdx.x == 0) __hip_atomic_store(&a.flags[ty], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+
+        // Subscribe and import: lane 0 of each wavefront holding halo rows polls
+        // the neighbour they come from; once its counter is seen the wavefront
+        // loads the neighbour's rows straight into its registers (device-scope
+        // loads issued after the counter load returned, so they see the rows
+        // the neighbour's acknowledged stores wrote). Lane i's dwords 2k, 2k+1
+        // are pair slot k*64 + i of a row.
+        const unsigned long long wanted = target + a.wait_ahead;
+        int ok = 1;
+        if (import_top || import_bottom) {
+            if (lane == 0) {
+                if (import_top) ok = wait_counter(&a.flags[ty - 1], wanted, a.error, spins) ? 1 : 0;
+                if (ok && import_bottom) ok = wait_counter(&a.flags[ty + 1], wanted, a.error, spins) ? 1 : 0;
+            }
+            ok = __builtin_amdgcn_readfirstlane(ok);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            if (ok) {
+                const unsigned long long* above =
+                    a.xbuf + (static_cast<size_t>(par) * a.tiles_y + (ty > 0 ? ty - 1 : 0)) * 2 * plane + plane;
+                const unsigned long long* below =
+                    a.xbuf + (static_cast<size_t>(par) * a.tiles_y + (ty + 1 < a.tiles_y ? ty + 1 : ty)) * 2 * plane;
+#pragma unroll
+                for (int r = 0; r < BR; ++r) {
+                    const int tr = w * BR + r;
+                    const bool top = import_top && tr < T;
+                    const bool bottom = import_bottom && tr >= tile_rows - T;
+                    if (top || bottom) {
+                        const unsigned long long* row =
+                            top ? above + static_cast<size_t>(tr) * RP
+                                : below + static_cast<size_t>(tr - (tile_rows - T)) * RP;
+#pragma unroll
+                        for (int k = 0; k < DW / 2; ++k) {
+                            const unsigned long long v = fetch_pair(row + k * 64 + lane);
+                            s[r][2 * k] = static_cast<uint32_t>(v) & colmask[2 * k];
+                            s[r][2 * k + 1] = static_cast<uint32_t>(v >> 32) & colmask[2 * k + 1];
+                        }
+                    }
+                }
+            } else if (lane == 0) {
+                *keep_going = 0;
+            }
         }
         __syncthreads();
         if (!*keep_going) return false;  // the whole workgroup leaves together; the error is recorded
-
-        // Import: all threads copy the neighbours' rows into LDS with wide
-        // independent loads (their latencies overlap), then each wavefront
-        // takes its halo rows from LDS. Both sides go in one pass when the
-        // stage holds 2T rows, else the upper side first, then the lower.
-        const unsigned long long* above =
-            a.xbuf + (static_cast<size_t>(par) * a.tiles_y + (ty > 0 ? ty - 1 : 0)) * 2 * plane + plane;
-        const unsigned long long* below =
-            a.xbuf + (static_cast<size_t>(par) * a.tiles_y + (ty + 1 < a.tiles_y ? ty + 1 : ty)) * 2 * plane;
-        const int passes = 2 * T <= STAGE_ROWS ? 1 : 2;
-        const int below_at = passes == 1 ? T * RP : 0;  // stage pair offset of the lower side
-#pragma nounroll
-        for (int pass = 0; pass < passes; ++pass) {
-            const bool take_above = (passes == 1 || pass == 0) && ty > 0;
-            const bool take_below = (passes == 1 || pass == 1) && ty + 1 < a.tiles_y;
-            {
-                // every load is issued before the first is used; slots are
-                // clamped to the plane so that no load needs a branch. The
-                // stage holds rows in natural order, 64*DW dwords each.
-                const int pairs = T * RP;
-                unsigned long long va[K], vb[K];
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const int i = min(static_cast<int>(threadIdx.x) + k * 64 * NW, pairs - 1);
-                    va[k] = fetch_pair(above + i);
-                    vb[k] = fetch_pair(below + i);
-                }
-                unsigned long long* stage = reinterpret_cast<unsigned long long*>(halo_stage);
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const int i = static_cast<int>(threadIdx.x) + k * 64 * NW;
-                    const int row = i / RP, slot = i % RP;
-                    const int at = row * RP + (slot % 64) * (DW / 2) + slot / 64;  // pair index in natural order
-                    if (i < pairs) {
-                        if (take_above) stage[at] = va[k];
-                        if (take_below) stage[below_at + at] = vb[k];
-                    }
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int r = 0; r < BR; ++r) {
-                const int tr = w * BR + r;
-                const bool top = take_above && tr < T;
-                const bool bottom = take_below && tr >= tile_rows - T;
-                if (top || bottom) {
-                    const uint32_t* row = halo_stage + 2 * (top ? tr * RP : below_at + (tr - (tile_rows - T)) * RP);
-#pragma unroll
-                    for (int d = 0; d < DW; ++d) s[r][d] = gx0 + d < g.x_dw ? row[gx0 + d] : 0u;
-                }
-            }
-            __syncthreads();  // the stage is refilled by the next pass or exchange
-        }
     }
     return true;
 }
 
-// Rows of 64*DW dwords staged per import: both sides of the deepest halo
-// (2 x kSlabMaxSteps) when that fits in LDS beside the edge rows, else one side.
-template <int DW, int NW>
-constexpr int slab_stage_rows() {
-    return sizeof(EdgeRows<DW, NW>) + 2 * kSlabMaxSteps * 64 * DW * sizeof(uint32_t) <= 156 * 1024
-               ? 2 * kSlabMaxSteps
-               : kSlabMaxSteps;
-}
-
 template <int DW, int BR, int NW, bool MASK_COLS>
 __global__ __launch_bounds__(64 * NW) void slab_kernel(SlabArgs a) {
-    constexpr int STAGE_ROWS = slab_stage_rows<DW, NW>();
     __shared__ EdgeRows<DW, NW> lds;
-    __shared__ __align__(16) uint32_t halo_stage[STAGE_ROWS * 64 * DW];
     __shared__ int keep_going;
     const TileGeo& g = a.geo;
     const int lane = threadIdx.x & 63;
@@ -721,10 +693,10 @@ __global__ __launch_bounds__(64 * NW) void slab_kernel(SlabArgs a) {
     const int row_lo = gy0 < 0 ? -gy0 : 0;
     const int row_hi = g.y_rows - gy0 < BR ? g.y_rows - gy0 : BR;
     const bool edge_y = y0 < 0 || y0 + tile_rows > g.y_rows;  // workgroup-uniform
-    const bool done = edge_y ? slab_blocks<DW, BR, NW, STAGE_ROWS, true, MASK_COLS>(a, s, lds, halo_stage, &keep_going,
-                                                                                     lane, w, gx0, row_lo, row_hi, colmask)
-                             : slab_blocks<DW, BR, NW, STAGE_ROWS, false, MASK_COLS>(
-                                   a, s, lds, halo_stage, &keep_going, lane, w, gx0, row_lo, row_hi, colmask);
+    const bool done = edge_y ? slab_blocks<DW, BR, NW, true, MASK_COLS>(a, s, lds, &keep_going, lane, w, row_lo,
+                                                                         row_hi, colmask)
+                             : slab_blocks<DW, BR, NW, false, MASK_COLS>(a, s, lds, &keep_going, lane, w, row_lo,
+                                                                          row_hi, colmask);
     if (!done) return;
 #pragma unroll
     for (int r = 0; r < BR; ++r) {
