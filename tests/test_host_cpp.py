@@ -21,6 +21,17 @@ def test_plugins_host_side():
     assert rc == 0, out
 
 
+def test_host_code_under_sanitizers():
+    """Parsers, loaders, packers and the oracle built with ASan + UBSan
+    (tests/cpp/test_host_sanitized.cpp): fuzzed expressions and flags end in
+    the documented exceptions, packers match the oracle, no sanitizer report."""
+    env = dict(os.environ, ASAN_OPTIONS="abort_on_error=0:halt_on_error=1", UBSAN_OPTIONS="print_stacktrace=1")
+    p = subprocess.run([os.path.join(ROOT, "tests", "cpp", "test_host_sanitized")], capture_output=True, text=True,
+                       timeout=240, env=env)
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "ok" in p.stdout
+
+
 def test_cli_rejects_bad_flags():
     rc, out = run([STENCILS, "--algorithm=gol-hip-rows-64", "--grid-dimensions-x=64"])
     assert rc != 0 and "required" in out
