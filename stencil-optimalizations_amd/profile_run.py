@@ -7,7 +7,10 @@
   python3 profile_run.py plain    16384^2 random, 100 generations with the plain kernel
   python3 profile_run.py active   16384^2 mostly-stable patches, 20000 generations, change tracking
                                   (the bench's mostly-stable configuration)
-  python3 profile_run.py batch    256 mixed grids (sides 64..2048, 100..500 generations), one batch run
+  python3 profile_run.py batch    the bench's batched workload: 4096 uneven grids (sides 64..2048,
+                                  100..5000 generations), built once, then two graph replays
+  python3 profile_run.py served   the bench's served stream (400 jobs, sides 128..8192, 100..50000
+                                  generations) submitted at once to a server (its capacity run)
   python3 profile_run.py copy     calibration: a torch device copy of 256 MiB, 10 times
                                   (reads and writes exactly 256 MiB per dispatch)
 """
@@ -21,9 +24,8 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
-import numpy as np  # noqa: E402
-
 import golhip  # noqa: E402
+import workloads  # noqa: E402
 
 
 def main(what: str) -> None:
@@ -57,16 +59,30 @@ def main(what: str) -> None:
         print(f"active: 20000 generations, {info.launches} launches, {ms:.2f} ms GPU, "
               f"{info.active_tiles_total} active tile-blocks of {info.tiles * info.tracked_blocks}", flush=True)
     elif what == "batch":
-        rng = np.random.default_rng(3)
+        specs = workloads.batched_uneven(4096)
+        inputs = workloads.make_inputs(specs)
         with golhip.Batch(device=0) as b:
-            for i in range(256):
-                side = int(64 * max(1, round(np.exp(rng.uniform(np.log(64), np.log(2048))) / 64)))
-                gens = int(rng.integers(100, 501))
-                b.add(golhip.fill_bernoulli(side, side, 7000 + i, float(rng.uniform(0.2, 0.5))), side, side, gens)
-            ms = b.run()
+            for s, w in zip(specs, inputs):
+                b.add(w, s.side, s.side, s.generations)
+            b.run()  # plans, uploads and records the graph
+            for _ in range(2):
+                ms = b.run()
+                print(f"batch: replay {ms:.2f} ms GPU", flush=True)
             info = b.info()
-        print(f"batch: {info.grids} grids, {info.rounds} rounds, {info.launches} launches, {ms:.2f} ms GPU",
-              flush=True)
+        print(f"batch: {info.grids} grids, {info.rounds} rounds, {info.launches} launches per replay", flush=True)
+    elif what == "served":
+        specs = workloads.served_stream(400, 1.0)
+        inputs = workloads.make_inputs(specs)
+        total = sum(s.cell_updates for s in specs)
+        with golhip.Server(device=0, hbm_budget_bytes=32 << 30) as srv:
+            t = time.perf_counter()
+            ids = [srv.submit(w, s.side, s.side, s.generations) for s, w in zip(specs, inputs)]
+            for i in ids:
+                srv.wait(i)
+            wall = time.perf_counter() - t
+            st = srv.stats()
+        print(f"served: {len(specs)} jobs in {wall:.3f} s, {total / wall:.3e} cell-updates/s, "
+              f"{st.rounds} rounds, {st.items} items", flush=True)
     elif what == "copy":
         import torch
 
