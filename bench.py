@@ -197,6 +197,17 @@ def leg_batched(n_grids: int, plain_sample: int) -> dict:
             if not np.array_equal(res, b.fetch(i)):
                 raise SystemExit(f"batched grid {i} differs from the same grid run alone")
             sample_updates += s.cell_updates
+        round_hashes = [hashlib.sha256(b.fetch(i).tobytes()).digest() for i in range(len(specs))]
+    # the same batch as persistent queue launches (GOL_BATCH_PERSISTENT)
+    with golhip.Batch(device=0, persistent=True) as q:
+        for s, w in zip(specs, inputs):
+            q.add(w, s.side, s.side, s.generations)
+        q.run()
+        queue_ms = min(q.run() for _ in range(2))
+        queue_info = q.info()
+        same = all(hashlib.sha256(q.fetch(i).tobytes()).digest() == round_hashes[i] for i in range(len(specs)))
+    if not same:
+        raise SystemExit("the persistent queue launch differs from the per-round launches")
     batched_rate = total / (batch_ms / 1e3)
     plain_rate = sample_updates / plain_wall
     alone_rate = sample_updates / alone_wall
@@ -216,6 +227,14 @@ def leg_batched(n_grids: int, plain_sample: int) -> dict:
         "speedup_vs_plain_per_grid": batched_rate / plain_rate,
         "speedup_vs_per_grid_temporal": batched_rate / alone_rate,
         "sample_bit_exact_alone_vs_batched": True,
+        "persistent_queue": {
+            "ms": queue_ms,
+            "cell_updates_per_s": total / (queue_ms / 1e3),
+            "launches": queue_info.launches,
+            "items": queue_info.items,
+            "vs_rounds": batch_ms / queue_ms,
+            "all_grids_identical_to_rounds": True,
+        },
         "input_generation_s": gen_s,
     }
 

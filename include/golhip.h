@@ -167,18 +167,28 @@ void gol_destroy(gol_handle* h);
  * result is bit-identical to running it alone through gol_step. ---- */
 typedef struct gol_batch gol_batch;
 
+/* gol_batch_config.flags */
+#define GOL_BATCH_PERSISTENT 1u /* one persistent launch per kernel shape present, its workgroups draining a
+                                   device work queue of (grid, tile, generations) items; a split grid's round
+                                   waits on the device for the grid's previous round. Without it: one launch
+                                   per round and kernel shape. */
+#define GOL_BATCH_WAIT_PROBE 2u /* testing the failure path of GOL_BATCH_PERSISTENT: every item that continues
+                                   a grid waits (256 polls at most) for a count no item reaches, so the run
+                                   stops and gol_batch_run reports GOL_ERR_RUNTIME */
+
 typedef struct gol_batch_config {
     int32_t device;             /* HIP device ordinal; -1 = current device */
     uint32_t round_generations; /* generations per round for grids split into tiles (1..32); 0 = 32 */
-    uint32_t use_graph;         /* 1: record all rounds once into a hipGraph and replay it */
-    uint32_t reserved;
+    uint32_t use_graph;         /* 1: record all launches once into a hipGraph and replay it */
+    uint32_t flags;             /* GOL_BATCH_* */
 } gol_batch_config;
 
 typedef struct gol_batch_info {
     uint32_t grids;
-    uint32_t rounds;            /* rounds of work items */
+    uint32_t rounds;            /* rounds of work items (persistent: the most rounds of one grid) */
     uint64_t items;             /* work items (grid tile x round) */
-    uint64_t launches;          /* kernel launches per run (inside the graph when use_graph) */
+    uint64_t launches;          /* kernel launches per run (inside the graph when use_graph); persistent:
+                                   one per kernel shape + one that clears the queue counters */
     uint64_t device_bytes;
     uint64_t cell_updates;      /* sum over grids of cells x generations */
 } gol_batch_info;

@@ -234,8 +234,15 @@ struct gol_batch {
         size_t first;  // index into the item array
         int count;
         int lane;
+        int workgroups = 0;  // persistent queue launch: its workgroups (0: one workgroup per item)
     };
     std::vector<Launch> launches;
+    // persistent queue launches (GOL_BATCH_PERSISTENT): one per launch key;
+    // qstate holds each launch's ticket counter, then each grid's completed
+    // items, then the error word
+    bool queue = false;
+    unsigned int* d_qstate = nullptr;
+    int qstate_words = 0;
     uint32_t rounds = 0;
     uint64_t items = 0;
     uint64_t cell_updates = 0;
@@ -257,6 +264,86 @@ struct Guard {
         if (prev >= 0 && hipGetDevice(&now) == hipSuccess && now != prev) (void)hipSetDevice(prev);
     }
 };
+
+// Generations one item of a one-tile grid advances in a persistent queue
+// launch: such a grid needs no halo, so one item runs all its generations
+// (longer runs are split into items that wait for each other).
+constexpr uint64_t kQueueItemGenerations = uint64_t{1} << 20;
+
+// Persistent queue launches (GOL_BATCH_PERSISTENT): one per launch key. A grid
+// that one tile spans is one item; a split grid is rounds of one item per
+// tile, round r of the grid waiting for the grid's r x tiles earlier items.
+// Each queue holds its items round by round, so an item always follows those
+// it waits for, the most costly first within a round. The launches go onto the
+// lanes smallest total work first, each with as many workgroups as can be
+// resident, capped at the most items of its queue that can run at once (the
+// largest round).
+int plan_queues(gol_batch* b, const std::map<LaunchKey, int>& lane_of, std::vector<int>& parity,
+                std::vector<WorkItem>* all) {
+    struct Entry {
+        int round;
+        double cost;
+        WorkItem item;
+    };
+    struct Queue {
+        std::vector<Entry> entries;
+        std::map<int, int> per_round;  // items of each round
+        int waves = 0;                 // narrow grids: the tallest grid's wavefronts
+        double work = 0.0;
+    };
+    std::map<LaunchKey, Queue> queues;
+    for (size_t i = 0; i < b->grids.size(); ++i) {
+        auto& g = b->grids[i];
+        const LaunchKey key = launch_key(g.shape);
+        Queue& q = queues[key];
+        q.waves = std::max(q.waves, g.shape.waves);
+        const int tiles = g.geom.tiles_x * g.geom.tiles_y;
+        const uint64_t step = tiles == 1 ? kQueueItemGenerations : static_cast<uint64_t>(g.chunk);
+        const uint64_t cells = static_cast<uint64_t>(g.x_dw * 32 * g.y_rows);
+        uint64_t left = g.generations;
+        for (int round = 0; left > 0; ++round) {
+            const int gens = static_cast<int>(std::min(left, step));
+            for (int t = 0; t < tiles; ++t)
+                q.entries.push_back(Entry{round, static_cast<double>(cells) / tiles * gens,
+                                          WorkItem{static_cast<int>(i), t, gens, parity[i], round * tiles}});
+            q.per_round[round] += tiles;
+            parity[i] ^= 1;
+            left -= static_cast<uint64_t>(gens);
+            b->cell_updates += cells * static_cast<uint64_t>(gens);
+            b->rounds = std::max(b->rounds, static_cast<uint32_t>(round + 1));
+        }
+        q.work += static_cast<double>(cells) * static_cast<double>(g.generations);
+    }
+    std::vector<std::pair<double, LaunchKey>> order;
+    for (const auto& kv : queues) order.push_back({kv.second.work, kv.first});
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<double, LaunchKey>& a, const std::pair<double, LaunchKey>& c) {
+                         return a.first < c.first;
+                     });
+    for (const auto& o : order) {
+        Queue& q = queues[o.second];
+        if (q.entries.empty()) continue;  // every grid of this key has 0 generations
+        std::stable_sort(q.entries.begin(), q.entries.end(), [](const Entry& a, const Entry& c) {
+            return a.round != c.round ? a.round < c.round : a.cost > c.cost;
+        });
+        const LaunchKey& key = o.second;
+        gol_batch::Launch L;
+        L.shape = TileShape{key.lane_dwords, key.band_rows, key.band_rows ? key.waves : q.waves};
+        L.first = all->size();
+        L.count = static_cast<int>(q.entries.size());
+        L.lane = lane_of.at(key);
+        int parallel = 0;
+        for (const auto& pr : q.per_round) parallel = std::max(parallel, pr.second);
+        const int capacity = queue_capacity(L.shape);
+        if (capacity <= 0)
+            return fail(GOL_ERR_RUNTIME, "the persistent batch kernel for shape (%d, %d, %d) cannot be resident",
+                        L.shape.lane_dwords, L.shape.band_rows, L.shape.waves);
+        L.workgroups = std::min(capacity, parallel);
+        for (const auto& en : q.entries) all->push_back(en.item);
+        b->launches.push_back(L);
+    }
+    return GOL_OK;
+}
 
 // Plan shapes, rounds and items; allocate the arena and upload everything.
 int build(gol_batch* b) {
@@ -321,6 +408,11 @@ int build(gol_batch* b) {
         std::vector<WorkItem> items;
         int waves = 0;  // narrow grids: the tallest grid's wavefronts
     };
+    if (b->queue) {
+        int rc = plan_queues(b, lane_of, parity, &all);
+        if (rc) return rc;
+        left.assign(left.size(), 0);  // every generation is in the queues
+    }
     for (;;) {
         std::map<LaunchKey, Group> by_shape;
         bool any = false;
@@ -370,6 +462,13 @@ int build(gol_batch* b) {
                                 b->stream)) != hipSuccess)
             return fail_hip(e, "batch items upload");
     }
+    if (b->queue) {
+        const size_t words = b->launches.size() + b->grids.size() + 1;
+        if (words > (1u << 30)) return fail(GOL_ERR_INVALID_ARGUMENT, "too many grids for a persistent batch");
+        b->qstate_words = static_cast<int>(words);
+        if ((e = hipMalloc(&b->d_qstate, words * sizeof(unsigned int))) != hipSuccess)
+            return fail_hip(e, "batch queue state");
+    }
     if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return fail_hip(e, "batch build");
     for (auto& g : b->grids) {
         g.host.clear();
@@ -379,11 +478,14 @@ int build(gol_batch* b) {
     return GOL_OK;
 }
 
-// Fork the side lanes from the batch stream, run every lane's rounds, join.
+// Fork the side lanes from the batch stream, run every lane's rounds (or
+// persistent queue launches), join.
 int enqueue_rounds(gol_batch* b) {
     bool used[gol_batch::kLanes] = {};
     for (const auto& L : b->launches) used[L.lane] = true;
     hipError_t e;
+    if (b->queue && (e = launch_zero(b->d_qstate, b->qstate_words, b->stream)) != hipSuccess)
+        return fail_hip(e, "batch queue reset");
     bool forked = false;
     for (int l = 1; l < gol_batch::kLanes; ++l) {
         if (!used[l]) continue;
@@ -391,10 +493,26 @@ int enqueue_rounds(gol_batch* b) {
         forked = true;
         if ((e = hipStreamWaitEvent(b->side[l - 1], b->fork_ev, 0)) != hipSuccess) return fail_hip(e, "batch fork");
     }
-    for (const auto& L : b->launches) {
+    const bool probe = (b->cfg.flags & GOL_BATCH_WAIT_PROBE) != 0;
+    const int n_launches = static_cast<int>(b->launches.size());
+    for (int k = 0; k < n_launches; ++k) {
+        const auto& L = b->launches[static_cast<size_t>(k)];
         hipStream_t st = L.lane == 0 ? b->stream : b->side[L.lane - 1];
-        if ((e = launch_items(L.shape, b->d_grids, b->d_items + L.first, L.count, st)) != hipSuccess)
+        if (L.workgroups > 0) {
+            QueueArgs q;
+            q.grids = b->d_grids;
+            q.items = b->d_items + L.first;
+            q.n_items = L.count;
+            q.spin_limit = probe ? 256 : 0;
+            q.next = b->d_qstate + k;
+            q.done = b->d_qstate + n_launches;
+            q.error = b->d_qstate + b->qstate_words - 1;
+            q.wait_ahead = probe ? (1u << 30) : 0u;
+            if ((e = launch_queue(L.shape, q, L.workgroups, st)) != hipSuccess)
+                return fail_hip(e, "persistent batch launch");
+        } else if ((e = launch_items(L.shape, b->d_grids, b->d_items + L.first, L.count, st)) != hipSuccess) {
             return fail_hip(e, "item_kernel launch");
+        }
     }
     for (int l = 1; l < gol_batch::kLanes; ++l) {
         if (!used[l]) continue;
@@ -419,11 +537,16 @@ int gol_batch_create(const gol_batch_config* config, gol_batch** out) {
     if (config) cfg = *config;
     if (cfg.round_generations > 32)
         return fail(GOL_ERR_INVALID_ARGUMENT, "round_generations must be at most 32 (the one-lane x halo)");
+    if (cfg.flags & ~(GOL_BATCH_PERSISTENT | GOL_BATCH_WAIT_PROBE))
+        return fail(GOL_ERR_INVALID_ARGUMENT, "unknown batch flags 0x%x", cfg.flags);
+    if ((cfg.flags & GOL_BATCH_WAIT_PROBE) && !(cfg.flags & GOL_BATCH_PERSISTENT))
+        return fail(GOL_ERR_INVALID_ARGUMENT, "GOL_BATCH_WAIT_PROBE applies to persistent batches only");
     int dev = cfg.device;
     hipError_t e;
     if (dev < 0 && (e = hipGetDevice(&dev)) != hipSuccess) return fail_hip(e, "hipGetDevice");
     gol_batch* b = new gol_batch();
     b->cfg = cfg;
+    b->queue = (cfg.flags & GOL_BATCH_PERSISTENT) != 0;
     b->device = dev;
     Guard g(dev);
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
@@ -511,6 +634,15 @@ int gol_batch_run(gol_batch* b, double* gpu_ms) {
     }
     if ((e = hipEventRecord(b->ev1, b->stream)) != hipSuccess) return fail_hip(e, "hipEventRecord");
     if ((e = hipEventSynchronize(b->ev1)) != hipSuccess) return fail_hip(e, "gol_batch_run");
+    if (b->queue && b->d_qstate) {
+        unsigned int err = 0;
+        if ((e = hipMemcpy(&err, b->d_qstate + b->qstate_words - 1, sizeof(err), hipMemcpyDeviceToHost)) !=
+            hipSuccess)
+            return fail_hip(e, "gol_batch_run (queue state)");
+        if (err)
+            return fail(GOL_ERR_RUNTIME, "persistent batch stopped: an item waited too long for its grid's earlier "
+                        "items (GOL_BATCH_PERSISTENT off selects one launch per round)");
+    }
     if (gpu_ms) {
         float ms = 0.f;
         if ((e = hipEventElapsedTime(&ms, b->ev0, b->ev1)) != hipSuccess) return fail_hip(e, "hipEventElapsedTime");
@@ -548,7 +680,7 @@ int gol_batch_get_info(gol_batch* b, gol_batch_info* info) {
     info->grids = static_cast<uint32_t>(b->grids.size());
     info->rounds = b->rounds;
     info->items = b->items;
-    info->launches = b->launches.size();
+    info->launches = b->launches.size() + (b->queue && b->built ? 1 : 0);  // + the queue reset
     info->device_bytes = b->arena_dwords * 4 + b->items * sizeof(WorkItem) + b->grids.size() * sizeof(BatchGrid);
     info->cell_updates = b->cell_updates;
     return GOL_OK;
@@ -566,6 +698,7 @@ void gol_batch_destroy(gol_batch* b) {
         if (b->arena) (void)hipFree(b->arena);
         if (b->d_grids) (void)hipFree(b->d_grids);
         if (b->d_items) (void)hipFree(b->d_items);
+        if (b->d_qstate) (void)hipFree(b->d_qstate);
         if (b->ev0) (void)hipEventDestroy(b->ev0);
         if (b->ev1) (void)hipEventDestroy(b->ev1);
         if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);

@@ -34,12 +34,15 @@ struct BatchGrid {
 };
 
 // One unit of batched work: advance tile `tile` of grid `grid` by `gens`
-// generations from buf[parity] into buf[parity ^ 1].
+// generations from buf[parity] into buf[parity ^ 1]. `after` is used by the
+// persistent queue launch only: the item starts once that many items of its
+// grid have completed in the run (the earlier rounds it reads or overwrites).
 struct WorkItem {
     int grid;
     int tile;
     int gens;
     int parity;
+    int after;
 };
 
 // Tile kernel shapes compiled into the library: (DW, BR, NW).
@@ -130,6 +133,27 @@ hipError_t launch_fine_tiles(const FineGeo& g, const uint32_t* src, uint32_t* ds
 // Batched grids: one workgroup per work item, all items of one tile shape.
 hipError_t launch_items(const TileShape& shape, const BatchGrid* grids, const WorkItem* items, int n_items,
                         hipStream_t stream);
+
+// Persistent batch launch: `workgroups` workgroups drain a device work queue
+// of items of one kernel shape (a narrow shape launches 64 * waves threads).
+// Every item must follow, in the queue, the items of its grid it waits for
+// (WorkItem::after). next and done start at 0; a wait that runs out of polls
+// raises *error, and every workgroup then leaves.
+struct QueueArgs {
+    const BatchGrid* grids;
+    const WorkItem* items;
+    int n_items;
+    int spin_limit;           // polls before a wait gives up; 0: the built-in limit
+    unsigned int* next;       // ticket counter
+    unsigned int* done;       // per grid: items completed in this run
+    unsigned int* error;
+    unsigned int wait_ahead;  // 0; testing the failure path: waits for counts no item reaches
+};
+hipError_t launch_queue(const TileShape& shape, const QueueArgs& q, int workgroups, hipStream_t stream);
+// Workgroups of the queue kernel for `shape` resident at once on the current device (0 if unknown).
+int queue_capacity(const TileShape& shape);
+// Sets n words to 0 (a kernel, so that it may be recorded into a graph).
+hipError_t launch_zero(unsigned int* words, int n, hipStream_t stream);
 
 // Persistent slab launch over a grid that one tile spans in x (tiles_x == 1).
 struct SlabArgs {

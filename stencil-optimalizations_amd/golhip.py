@@ -74,6 +74,9 @@ class GolConfig(ctypes.Structure):
 
 FLAG_NO_PERSISTENT = 1
 FLAG_SLAB_WAIT_PROBE = 2
+# gol_batch_config.flags
+BATCH_PERSISTENT = 1
+BATCH_WAIT_PROBE = 2
 
 LAYOUT_ROWS, LAYOUT_COLUMNS, LAYOUT_TILES = 0, 1, 2
 
@@ -101,7 +104,7 @@ class GolBatchConfig(ctypes.Structure):
         ("device", ctypes.c_int32),
         ("round_generations", ctypes.c_uint32),
         ("use_graph", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("flags", ctypes.c_uint32),
     ]
 
 
@@ -313,9 +316,11 @@ class Grid:
 class Batch:
     """Many grids advanced together (gol_batch_*)."""
 
-    def __init__(self, device: int = -1, round_generations: int = 0, use_graph: bool = True):
+    def __init__(self, device: int = -1, round_generations: int = 0, use_graph: bool = True,
+                 persistent: bool = False, flags: int = 0):
         self.lib = load_library()
-        cfg = GolBatchConfig(device, round_generations, 1 if use_graph else 0, 0)
+        flags |= BATCH_PERSISTENT if persistent else 0
+        cfg = GolBatchConfig(device, round_generations, 1 if use_graph else 0, flags)
         h = ctypes.c_void_p()
         _check(self.lib.gol_batch_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h

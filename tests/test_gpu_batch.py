@@ -153,3 +153,70 @@ def test_autotune_installs_measured_choices():
                 assert np.array_equal(b.fetch(i), ref), (x, y, gens)
     finally:
         golhip.batch_tuning_reset()
+
+
+# ---- persistent queue launch (GOL_BATCH_PERSISTENT) ----
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_persistent_batch_against_oracle(use_graph):
+    """One persistent launch per kernel shape drains a device work queue; every
+    grid, including split grids whose rounds wait for each other on the device,
+    is bit-exact against the oracle, and a re-run restarts from the initial state."""
+    specs = mixed_specs(40, 2, max_side=1536, max_gens=400)
+    with golhip.Batch(use_graph=use_graph, persistent=True, round_generations=12) as b:
+        for x, y, gens, p, seed in specs:
+            b.add(golhip.fill_bernoulli(x, y, seed, p), x, y, gens)
+        note("persistent batch of", len(specs), "graph", use_graph)
+        b.run()
+        info = b.info()
+        assert info.grids == len(specs)
+        for i, (x, y, gens, p, seed) in enumerate(specs):
+            ref = ol.bitpar_run(golhip.fill_bernoulli(x, y, seed, p), x, y, gens)
+            assert np.array_equal(b.fetch(i), ref), (x, y, gens)
+        note("persistent batch re-run")
+        b.run()
+        for i in (0, 5, 17):
+            x, y, gens, p, seed = specs[i]
+            assert np.array_equal(b.fetch(i), ol.bitpar_run(golhip.fill_bernoulli(x, y, seed, p), x, y, gens))
+
+
+def test_persistent_split_and_narrow_grids():
+    """Split grids with many dependent rounds next to narrow grids advanced in
+    one item each, and 0-generation grids: persistent equals per-round launches
+    equals the oracle."""
+    specs = [(2048, 2048, 150, 0.5, 21), (1088, 1600, 97, 0.3, 22), (64, 2048, 150, 0.4, 23),
+             (2048, 64, 64, 0.5, 24), (1984, 1216, 33, 0.45, 25), (4160, 1000, 20, 0.4, 26),
+             (512, 512, 700, 0.35, 27), (192, 63, 1000, 0.3, 28), (448, 255, 0, 0.3, 29), (256, 1, 5, 0.5, 30)]
+    results = {}
+    for persistent in (False, True):
+        with golhip.Batch(round_generations=8, persistent=persistent) as b:
+            for x, y, gens, p, seed in specs:
+                b.add(golhip.fill_bernoulli(x, y, seed, p), x, y, gens)
+            note("split and narrow, persistent", persistent)
+            b.run()
+            results[persistent] = [b.fetch(i) for i in range(len(specs))]
+    for i, (x, y, gens, p, seed) in enumerate(specs):
+        ref = ol.bitpar_run(golhip.fill_bernoulli(x, y, seed, p), x, y, gens)
+        assert np.array_equal(results[True][i], ref), (x, y, gens)
+        assert np.array_equal(results[False][i], results[True][i]), (x, y, gens)
+
+
+def test_persistent_wait_timeout_is_reported():
+    """GOL_BATCH_WAIT_PROBE makes every item that continues a grid wait, with a
+    short spin limit, for a count no item reaches: the run stops, every kernel
+    ends, gol_batch_run reports GOL_ERR_RUNTIME, and a new batch runs normally."""
+    specs = [(2048, 1024, 40, 0.5, 41), (1024, 1024, 30, 0.4, 42), (128, 128, 50, 0.3, 43)]
+    with golhip.Batch(round_generations=8, persistent=True, flags=golhip.BATCH_WAIT_PROBE) as b:
+        for x, y, gens, p, seed in specs:
+            b.add(golhip.fill_bernoulli(x, y, seed, p), x, y, gens)
+        note("persistent wait probe")
+        with pytest.raises(golhip.GolError) as ei:
+            b.run()
+        assert ei.value.code == golhip.GOL_ERR_RUNTIME
+    with golhip.Batch(round_generations=8, persistent=True) as b:
+        for x, y, gens, p, seed in specs:
+            b.add(golhip.fill_bernoulli(x, y, seed, p), x, y, gens)
+        note("persistent batch after the probe")
+        b.run()
+        for i, (x, y, gens, p, seed) in enumerate(specs):
+            assert np.array_equal(b.fetch(i), ol.bitpar_run(golhip.fill_bernoulli(x, y, seed, p), x, y, gens))
