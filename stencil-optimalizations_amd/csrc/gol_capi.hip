@@ -50,28 +50,55 @@ TileShape default_shape(int64_t x_dw, int64_t y_rows) {
     return TileShape{8, 10, 8};
 }
 
-// Time model of one temporal launch on MI355X, used to choose a tile shape
-// and generations per launch for a single grid (measured on 16384^2: within
-// about 5% of the launch times of every shape tried):
-//   compute: each CU advances one tile at a time; a dword-generation costs
-//            about 13.5 VALU ops at 64 lanes per clock, 2.4 GHz;
+// Time model of temporal launches on MI355X, used to choose a tile shape,
+// generations per launch and the launch form for a single grid. On 16384^2
+// random grids it stays within about 10 % of the measured launch times of every
+// shape and depth tried:
+//   compute: each CU advances its tiles one after another; a dword-generation
+//            costs about 12.7 + 8 / band_rows VALU ops (13.5 at 10 rows: the
+//            rows just above and below a band are summed once per band) at
+//            64 lanes per clock, 2.4 GHz;
 //   memory:  every tile loads its rows with halo and stores its interior, at
-//            about 5 TB/s of HBM bandwidth, not overlapped with compute;
-//   launch:  about 4 us between dependent launches.
-double model_us_per_generation(const TileShape& s, const TileGeometry& g, int steps) {
+//            about 5 TB/s of HBM bandwidth. With one tile per CU at a time
+//            (occupancy 1) this does not overlap compute; with several
+//            resident per CU and more tiles than CUs, other tiles' compute hides
+//            it except for about one tile's share;
+//   launch:  about 4 us between dependent launches;
+//   persistent slabs: one tile per CU for the whole run, no per-block load,
+//            store or launch, about 8 us of halo exchange per block.
+double model_us_per_generation(const TileShape& s, const TileGeometry& g, int steps, int occupancy,
+                               bool persistent) {
     const double cus = 256.0;
     const double tiles = static_cast<double>(g.tiles_x) * g.tiles_y;
     const double waves_of_tiles = std::ceil(tiles / cus);
     const double dwords_per_tile = static_cast<double>(s.band_rows) * s.waves * 64.0 * s.lane_dwords;
-    const double compute_us = waves_of_tiles * dwords_per_tile * steps * 13.5 / 64.0 / 2400.0;
+    const double ops = 12.7 + 8.0 / s.band_rows;
+    const double compute_us = waves_of_tiles * dwords_per_tile * steps * ops / 64.0 / 2400.0;
+    if (persistent) return (compute_us + 8.0) / steps;
     const double bytes = tiles * (static_cast<double>(g.tile_rows) + g.stride_rows) * 64.0 * s.lane_dwords * 4.0;
     const double memory_us = bytes / 5.0e6;
-    return (compute_us + memory_us + 4.0) / steps;
+    const double busy = occupancy >= 2 && waves_of_tiles >= 2
+                            ? std::max(compute_us, memory_us) + memory_us / waves_of_tiles
+                            : compute_us + memory_us;
+    return (busy + 4.0) / steps;
 }
 
-// Shape and generations per launch with the lowest modelled time per generation.
+// Whether the persistent slab launch takes this shape and depth for the grid:
+// one tile spans the width, every slab is resident at once, the interiors are
+// at least as tall as the exchanged halos and blocks are at most kSlabMaxSteps
+// generations.
+bool slab_fits(const TileShape& s, const TileGeometry& g, int64_t x_dw, int steps) {
+    if (g.tiles_x != 1 || g.tiles_y < 2 || g.stride_rows < steps || steps > kSlabMaxSteps || s.lane_dwords % 2 != 0 ||
+        x_dw > 64 * s.lane_dwords)
+        return false;
+    int capacity = 0;
+    return slab_capacity(s, 64 * s.lane_dwords > x_dw, &capacity) && capacity >= g.tiles_y;
+}
+
+// Shape and generations per launch with the lowest modelled time per
+// generation; allow_persistent prices shapes the slab launch takes as slabs.
 bool choose_temporal(int64_t x_dw, int64_t y_rows, int fixed_dw, int fixed_br, int fixed_nw, int fixed_steps,
-                     TileShape* best_shape, int* best_steps) {
+                     bool allow_persistent, TileShape* best_shape, int* best_steps) {
     static const int kSteps[] = {2, 4, 6, 8, 10, 12, 16, 24, 32};
     double best = 1e300;
     bool found = false;
@@ -81,14 +108,16 @@ bool choose_temporal(int64_t x_dw, int64_t y_rows, int fixed_dw, int fixed_br, i
             (fixed_nw && s.waves != fixed_nw))
             continue;
         const bool one_tile = x_dw <= 64 * s.lane_dwords && y_rows <= s.band_rows * s.waves;
+        const int occupancy = tile_occupancy(s);
         for (int steps : kSteps) {
             if (fixed_steps && steps != fixed_steps) continue;
             TileGeometry g;
             const char* why = nullptr;
             const int plan_steps = one_tile ? 1 : steps;
             if (!plan_tiles(s, x_dw, y_rows, plan_steps, &g, &why)) continue;
+            const bool slab = allow_persistent && !one_tile && slab_fits(s, g, x_dw, steps);
             // narrow rows on wide lanes waste lanes: charge the full tile width
-            double t = model_us_per_generation(s, g, one_tile ? 64 : steps);
+            const double t = model_us_per_generation(s, g, one_tile ? 64 : steps, occupancy, slab);
             if (t < best) {
                 best = t;
                 *best_shape = s;
@@ -224,7 +253,8 @@ int plan(gol_handle* h) {
         TileShape m;
         int ms = 0;
         if (choose_temporal(h->x_dw, h->y_rows, static_cast<int>(c.lane_dwords), static_cast<int>(c.band_rows),
-                            static_cast<int>(c.waves), static_cast<int>(c.temporal_steps), &m, &ms)) {
+                            static_cast<int>(c.waves), static_cast<int>(c.temporal_steps),
+                            !(c.flags & GOL_FLAG_NO_PERSISTENT), &m, &ms)) {
             s = m;
             modelled_steps = ms;
         }
@@ -249,12 +279,8 @@ int plan(gol_handle* h) {
     // persistent slabs: every tile resident at once, one tile across the width,
     // interiors at least as tall as the exchanged halos
     if (h->kernel == GOL_KERNEL_TEMPORAL && !(c.flags & GOL_FLAG_NO_PERSISTENT) && !h->one_tile &&
-        h->geom.tiles_x == 1 && h->geom.tiles_y > 1 && h->geom.stride_rows >= steps && steps <= 8 &&
-        h->x_dw <= 64 * s.lane_dwords && s.lane_dwords % 2 == 0) {
-        int capacity = 0;
-        if (slab_capacity(s, 64 * s.lane_dwords > h->x_dw, &capacity) && capacity >= h->geom.tiles_y)
-            h->persistent = true;
-    }
+        slab_fits(s, h->geom, h->x_dw, steps))
+        h->persistent = true;
     return GOL_OK;
 }
 

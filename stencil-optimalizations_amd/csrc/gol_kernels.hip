@@ -691,8 +691,6 @@ __global__ void zero_kernel(unsigned int* words, int n) {
 // the launch always ends.
 // ---------------------------------------------------------------------------
 constexpr int kSlabSpinLimit = 1 << 20;  // polls before a wait gives up (SlabArgs::spin_limit 0)
-// Deepest halo (generations per block) a slab launch takes.
-constexpr int kSlabMaxSteps = 8;
 
 // Relaxed polling of a neighbour's block counter.
 __device__ bool wait_counter(const unsigned long long* p, unsigned long long target, unsigned int* error,
@@ -936,6 +934,23 @@ hipError_t launch_tiles(const TileGeometry& g, const uint32_t* src, uint32_t* ds
     GOLHIP_FOR_EACH_SHAPE(GOLHIP_LAUNCH)
 #undef GOLHIP_LAUNCH
     return hipErrorInvalidValue;
+}
+
+static const void* tile_function(const TileShape& shape) {
+#define GOLHIP_TILE_FN(DW, BR, NW)                                               \
+    if (shape.lane_dwords == DW && shape.band_rows == BR && shape.waves == NW)   \
+        return reinterpret_cast<const void*>(&tile_kernel<DW, BR, NW>);
+    GOLHIP_FOR_EACH_SHAPE(GOLHIP_TILE_FN)
+#undef GOLHIP_TILE_FN
+    return nullptr;
+}
+
+int tile_occupancy(const TileShape& shape) {
+    const void* f = tile_function(shape);
+    int per_cu = 0;
+    if (!f || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 64 * shape.waves, 0) != hipSuccess)
+        return 0;
+    return per_cu;
 }
 
 static const void* fine_function(int dw) {

@@ -90,6 +90,9 @@ hipError_t launch_tiles(const TileGeometry& g, const uint32_t* src, uint32_t* ds
 
 TileGeo to_device_geo(const TileGeometry& g, int64_t x_dw, int64_t y_rows);
 
+// Workgroups of the tile kernel for `shape` one CU holds at once on the current device (0 if unknown).
+int tile_occupancy(const TileShape& shape);
+
 // Change tracking over small tiles: one wavefront per tile of tile_dwords
 // (4 or 8: 128 or 256 cells) x tile_rows rows, lane l holding row y0 + l of
 // the tile's rows and halo_rows (= T) halo rows above and below:
@@ -156,6 +159,7 @@ int queue_capacity(const TileShape& shape);
 hipError_t launch_zero(unsigned int* words, int n, hipStream_t stream);
 
 // Persistent slab launch over a grid that one tile spans in x (tiles_x == 1).
+constexpr int kSlabMaxSteps = 8;  // deepest halo (generations per block) a slab launch takes
 struct SlabArgs {
     const uint32_t* src;
     uint32_t* dst;
