@@ -220,3 +220,36 @@ def test_persistent_wait_timeout_is_reported():
         b.run()
         for i, (x, y, gens, p, seed) in enumerate(specs):
             assert np.array_equal(b.fetch(i), ol.bitpar_run(golhip.fill_bernoulli(x, y, seed, p), x, y, gens))
+
+
+def test_persistent_chained_items_of_one_tile_grids():
+    """A grid one tile spans is one queue item of up to 2^20 generations; longer
+    runs become chained items, each waiting on the device for the one before.
+    Narrow and one-tile grids past that length equal the per-round launches
+    and the oracle."""
+    gens = (1 << 20) + 777
+    specs = [(64, 64, gens, 0.3, 51), (128, 40, gens, 0.4, 52), (2048, 64, 3000, 0.5, 53)]
+    results = {}
+    for persistent in (False, True):
+        with golhip.Batch(persistent=persistent) as b:
+            for x, y, g, p, seed in specs:
+                b.add(golhip.fill_bernoulli(x, y, seed, p), x, y, g)
+            note("chained items, persistent", persistent)
+            b.run()
+            results[persistent] = [b.fetch(i) for i in range(len(specs))]
+    for i, (x, y, g, p, seed) in enumerate(specs):
+        assert np.array_equal(results[True][i], results[False][i]), (x, y, g)
+    x, y, g, p, seed = specs[0]
+    assert np.array_equal(results[True][0], ol.bitpar_run(golhip.fill_bernoulli(x, y, seed, p), x, y, g))
+
+
+def test_persistent_batch_of_zero_generation_grids():
+    """No items at all: nothing is launched and every grid keeps its initial state."""
+    with golhip.Batch(persistent=True) as b:
+        words = [golhip.fill_bernoulli(256, 128, 60 + i, 0.4) for i in range(3)]
+        for w in words:
+            b.add(w, 256, 128, 0)
+        note("persistent batch of 0-generation grids")
+        b.run()
+        for i, w in enumerate(words):
+            assert np.array_equal(b.fetch(i), w)
