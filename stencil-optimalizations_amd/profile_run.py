@@ -9,6 +9,8 @@
                                   (the bench's mostly-stable configuration)
   python3 profile_run.py batch    the bench's batched workload: 4096 uneven grids (sides 64..2048,
                                   100..5000 generations), built once, then two graph replays
+  python3 profile_run.py batch_queue  the same batch as persistent queue launches (one per kernel
+                                  shape), built once, then two replays
   python3 profile_run.py served   the bench's served stream (400 jobs, sides 128..8192, 100..50000
                                   generations) submitted at once to a server (its capacity run)
   python3 profile_run.py copy     calibration: a torch device copy of 256 MiB, 10 times
@@ -58,10 +60,10 @@ def main(what: str) -> None:
             info = g.info()
         print(f"active: 20000 generations, {info.launches} launches, {ms:.2f} ms GPU, "
               f"{info.active_tiles_total} active tile-blocks of {info.tiles * info.tracked_blocks}", flush=True)
-    elif what == "batch":
+    elif what in ("batch", "batch_queue"):
         specs = workloads.batched_uneven(4096)
         inputs = workloads.make_inputs(specs)
-        with golhip.Batch(device=0) as b:
+        with golhip.Batch(device=0, persistent=what == "batch_queue") as b:
             for s, w in zip(specs, inputs):
                 b.add(w, s.side, s.side, s.generations)
             b.run()  # plans, uploads and records the graph
