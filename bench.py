@@ -313,6 +313,25 @@ def leg_served(n_jobs: int, load: float) -> dict:
             g.step(s.generations)
             if not np.array_equal(g.download(), results[i]):
                 raise SystemExit(f"served job {i} differs from the same job run alone")
+    # the served jobs as plain per-grid launches (one launch per generation, one
+    # job after another), on a seeded sample; each result equals the served one
+    rng = np.random.Generator(np.random.PCG64(11))
+    plain_sample = sorted(rng.choice(n_jobs, size=min(24, n_jobs), replace=False).tolist())
+    plain_updates = 0
+    plain_wall = 0.0
+    for i in plain_sample:
+        s = specs[i]
+        with golhip.Grid(s.side, s.side, kernel=golhip.KERNEL_PLAIN, device=0) as g:
+            g.upload(inputs[i])
+            g.sync()
+            t = time.perf_counter()
+            g.step(s.generations)
+            g.sync()
+            plain_wall += time.perf_counter() - t
+            if not np.array_equal(g.download(), results[i]):
+                raise SystemExit(f"served job {i} differs from the same job run with plain launches")
+        plain_updates += s.cell_updates
+    plain_rate = plain_updates / plain_wall
     lat_sorted = sorted(lat)
     return {
         "workload": f"{n_jobs} jobs, Poisson arrivals at {rate:.1f}/s (load {load:.2f} of the server's "
@@ -330,6 +349,10 @@ def leg_served(n_jobs: int, load: float) -> dict:
         "resident_bytes_peak": stats.resident_bytes_peak,
         "sample_bit_exact_alone_vs_served": len(sample),
         "all_jobs_identical_paced_vs_all_at_once": True,
+        "plain_per_job_sample": len(plain_sample),
+        "plain_per_job_cell_updates_per_s": plain_rate,
+        "capacity_vs_plain_per_job": capacity / plain_rate,
+        "paced_vs_plain_per_job": (total / wall) / plain_rate,
     }
 
 
@@ -491,6 +514,7 @@ def main() -> int:
         result["served"] = leg_served(args.served_jobs, args.served_load)
         result["latency_p50_ms"] = result["served"]["latency_p50_ms"]
         result["latency_p99_ms"] = result["served"]["latency_p99_ms"]
+        result["served_vs_plain_speedup"] = result["served"]["capacity_vs_plain_per_job"]
     if not args.no_cpu_baseline:
         log("timing the CPU baseline sample ...")
         result["cpu_baseline"] = cpu_baseline(args.cpu_sample_generations)
