@@ -21,6 +21,12 @@
 // lanes run their rounds independently between one fork and one join, and a
 // small shape fills the CUs a large one leaves idle. All rounds are recorded
 // once into a hipGraph and replayed, so a whole batch is one graph launch.
+//
+// With GOL_BATCH_PERSISTENT the same items run as one persistent launch per
+// launch key instead (plan_queues below): its workgroups drain the key's
+// queue, and a split grid's round waits on the device for the grid's previous
+// round. The results are the same bit for bit; on the batched workload the
+// per-round graph is faster, so it stays the default.
 
 #include <hip/hip_runtime.h>
 
