@@ -857,7 +857,7 @@ __global__ __launch_bounds__(64 * NW) void slab_kernel(SlabArgs a) {
 
 const TileShape kTileShapes[] = {
     {8, 8, 8}, {8, 10, 8}, {8, 5, 16}, {8, 4, 16}, {4, 8, 8}, {4, 8, 16}, {2, 16, 8}, {1, 8, 8}, {1, 16, 8},
-    {1, 32, 16},
+    {1, 24, 8}, {1, 32, 8}, {1, 32, 16},
 };
 
 TileGeo to_device_geo(const TileGeometry& g, int64_t x_dw, int64_t y_rows) {
@@ -920,6 +920,8 @@ bool plan_tiles(const TileShape& shape, int64_t x_dw, int64_t y_rows, int max_ge
     X(2, 16, 8)                    \
     X(1, 8, 8)                     \
     X(1, 16, 8)                    \
+    X(1, 24, 8)                    \
+    X(1, 32, 8)                    \
     X(1, 32, 16)
 
 hipError_t launch_tiles(const TileGeometry& g, const uint32_t* src, uint32_t* dst, int64_t x_dw, int64_t y_rows,

@@ -47,7 +47,7 @@ JOB_QUEUED, JOB_RESIDENT, JOB_DRAINING, JOB_DONE, JOB_FAILED = range(5)
 
 # tile shapes compiled into the library: (dwords per lane, rows per wavefront, wavefronts)
 TILE_SHAPES = ((8, 8, 8), (8, 10, 8), (8, 5, 16), (8, 4, 16), (4, 8, 8), (4, 8, 16), (2, 16, 8), (1, 8, 8),
-               (1, 16, 8), (1, 32, 16))
+               (1, 16, 8), (1, 24, 8), (1, 32, 8), (1, 32, 16))
 
 
 class GolError(RuntimeError):

@@ -26,8 +26,8 @@ KERNELS = [
     dict(kernel=golhip.KERNEL_TEMPORAL, temporal_steps=1),
     dict(kernel=golhip.KERNEL_TEMPORAL, temporal_steps=5),
 ]
-SHAPES = [(1, 8, 8), (1, 16, 8), (1, 32, 16), (2, 16, 8), (4, 8, 8), (4, 8, 16), (8, 4, 16), (8, 5, 16), (8, 8, 8),
-          (8, 10, 8)]
+SHAPES = [(1, 8, 8), (1, 16, 8), (1, 24, 8), (1, 32, 8), (1, 32, 16), (2, 16, 8), (4, 8, 8), (4, 8, 16), (8, 4, 16),
+          (8, 5, 16), (8, 8, 8), (8, 10, 8)]
 
 
 def note(*a):
