@@ -168,10 +168,12 @@ def leg_batched(n_grids: int, plain_sample: int) -> dict:
     inputs = workloads.make_inputs(specs)
     gen_s = time.perf_counter() - t0
     total = sum(s.cell_updates for s in specs)
-    with golhip.Batch(device=0) as b:
+    # the first run plans the batch under the autotuner's per-class choices and
+    # under the built-in model, times both on this workload and keeps the faster
+    with golhip.Batch(device=0, flags=golhip.BATCH_SELECT_PLAN) as b:
         for s, w in zip(specs, inputs):
             b.add(w, s.side, s.side, s.generations)
-        first_ms = b.run()  # builds, uploads and records the graph
+        first_ms = b.run()  # builds, uploads, selects the plan and records the graph
         runs = [b.run() for _ in range(2)]
         info = b.info()
         batch_ms = min(runs)
@@ -199,7 +201,7 @@ def leg_batched(n_grids: int, plain_sample: int) -> dict:
             sample_updates += s.cell_updates
         round_hashes = [hashlib.sha256(b.fetch(i).tobytes()).digest() for i in range(len(specs))]
     # the same batch as persistent queue launches (GOL_BATCH_PERSISTENT)
-    with golhip.Batch(device=0, persistent=True) as q:
+    with golhip.Batch(device=0, persistent=True, flags=golhip.BATCH_SELECT_PLAN) as q:
         for s, w in zip(specs, inputs):
             q.add(w, s.side, s.side, s.generations)
         q.run()
@@ -217,9 +219,11 @@ def leg_batched(n_grids: int, plain_sample: int) -> dict:
         "cell_updates": total,
         "batched_ms": batch_ms,
         "batched_cell_updates_per_s": batched_rate,
-        "first_run_ms_incl_build": first_ms,
+        "first_run_ms_incl_build_and_plan_selection": first_ms,
         "rounds": info.rounds,
         "launches_in_graph": info.launches,
+        "plan": {"kept": "built-in model" if info.plan_builtin else "autotuned per-class choices",
+                 "autotuned_ms": info.plan_tuned_ms, "builtin_ms": info.plan_builtin_ms},
         "plain_sample_grids": len(sample),
         "plain_per_grid_cell_updates_per_s": plain_rate,
         "plain_per_grid_gpu_cell_updates_per_s": sample_updates / (plain_gpu / 1e3),
@@ -233,6 +237,7 @@ def leg_batched(n_grids: int, plain_sample: int) -> dict:
             "launches": queue_info.launches,
             "items": queue_info.items,
             "vs_rounds": batch_ms / queue_ms,
+            "plan_kept": "built-in model" if queue_info.plan_builtin else "autotuned per-class choices",
             "all_grids_identical_to_rounds": True,
         },
         "input_generation_s": gen_s,

@@ -175,6 +175,9 @@ typedef struct gol_batch gol_batch;
 #define GOL_BATCH_WAIT_PROBE 2u /* testing the failure path of GOL_BATCH_PERSISTENT: every item that continues
                                    a grid waits (256 polls at most) for a count no item reaches, so the run
                                    stops and gol_batch_run reports GOL_ERR_RUNTIME */
+#define GOL_BATCH_SELECT_PLAN 4u /* the first gol_batch_run plans the batch twice, with the autotuner's
+                                    measured per-size-class choices and with the built-in model, times one
+                                    run of each and keeps the faster plan for every later run */
 
 typedef struct gol_batch_config {
     int32_t device;             /* HIP device ordinal; -1 = current device */
@@ -191,6 +194,10 @@ typedef struct gol_batch_info {
                                    one per kernel shape + one that clears the queue counters */
     uint64_t device_bytes;
     uint64_t cell_updates;      /* sum over grids of cells x generations */
+    uint32_t plan_builtin;      /* GOL_BATCH_SELECT_PLAN: 1 if the built-in model's plan was kept */
+    uint32_t reserved;
+    double plan_tuned_ms;       /* GOL_BATCH_SELECT_PLAN: timed run under the measured choices */
+    double plan_builtin_ms;     /* GOL_BATCH_SELECT_PLAN: timed run under the built-in model */
 } gol_batch_info;
 
 /* config may be NULL (current device, 32 generations per round, graph on). */

@@ -110,15 +110,18 @@ TileShape batch_shape(int64_t x_dw, int64_t y_rows) {
 
 // Cost per generation of advancing a whole grid split into tiles, in CU
 // microseconds, when many grids share the GPU (no wave quantization): the
-// compute of every tile row (about 13.5 VALU ops per dword at 64 lanes per
-// clock, 2.4 GHz) plus, once per launch, loading each tile with its halo and
-// storing its interior (about 5 TB/s shared by 256 CUs).
-static double split_cost_per_generation(const TileShape& s, const TileGeometry& g, int steps) {
+// compute of every tile row (about 12.7 + 8 / band_rows VALU ops per dword at
+// 64 lanes per clock, 2.4 GHz) and, once per round, loading each tile with its
+// halo and storing its interior (about 5 TB/s shared by 256 CUs). With
+// `occupancy` of two or more workgroups per CU, one tile's loads and stores
+// overlap another's compute (1: priced on top of compute).
+static double split_cost_per_generation(const TileShape& s, const TileGeometry& g, int steps, int occupancy) {
     const double tiles = static_cast<double>(g.tiles_x) * g.tiles_y;
     const double width = 64.0 * s.lane_dwords;
-    const double compute = tiles * g.tile_rows * width * 13.5 / 64.0 / 2400.0;
-    const double memory = tiles * (static_cast<double>(g.tile_rows) + g.stride_rows) * width * 4.0 * 256.0 / 5.0e6;
-    return compute + memory / steps;
+    const double compute = tiles * g.tile_rows * width * (12.7 + 8.0 / s.band_rows) / 64.0 / 2400.0;
+    const double memory =
+        tiles * (static_cast<double>(g.tile_rows) + g.stride_rows) * width * 4.0 * 256.0 / 5.0e6 / steps;
+    return occupancy >= 2 ? std::max(compute, memory) : compute + memory;
 }
 
 // The given choice, if it applies to the grid: one-tile shapes advance a
@@ -150,8 +153,9 @@ static bool plan_choice(const BatchChoice& c, int64_t x_dw, int64_t y_rows, int 
     return true;
 }
 
-int plan_batch_grid(int64_t x_dw, int64_t y_rows, int round_generations, bool use_tuned, TileShape* shape,
+int plan_batch_grid(int64_t x_dw, int64_t y_rows, int round_generations, Planner planner, TileShape* shape,
                     TileGeometry* geom, int* per_round) {
+    const bool use_tuned = planner == Planner::kBatch;
     constexpr int kBigTileRows = 512;
     const char* why = nullptr;
     // a choice being measured, then a measured choice for the size class
@@ -188,10 +192,11 @@ int plan_batch_grid(int64_t x_dw, int64_t y_rows, int round_generations, bool us
     double best = 1e300;
     for (int i = 0; i < kNumTileShapes; ++i) {
         const TileShape& s = kTileShapes[i];
+        const int occupancy = tile_occupancy(s);
         for (int steps = round_generations; steps >= 1; --steps) {
             TileGeometry g;
             if (!plan_tiles(s, x_dw, y_rows, steps, &g, &why)) continue;
-            const double c = split_cost_per_generation(s, g, steps);
+            const double c = split_cost_per_generation(s, g, steps, planner == Planner::kBatch ? occupancy : 1);
             if (c < best) {
                 best = c;
                 *shape = s;
@@ -217,7 +222,8 @@ struct BatchEntry {
     TileShape shape{1, 16, 8};
     TileGeometry geom{};
     int chunk = 0;               // generations per round
-    int final_parity = 0;
+    int final_parity = 0;        // buffer holding the result after a run of the current plan
+    int result_parity = 0;       // buffer holding the result of the last completed run
 };
 
 struct gol_batch {
@@ -249,6 +255,11 @@ struct gol_batch {
     bool queue = false;
     unsigned int* d_qstate = nullptr;
     int qstate_words = 0;
+    bool dirty = false;                // a run changed the buffers since the initial states were restored
+    bool result_parity_valid = false;  // a run completed since then: BatchEntry::result_parity holds
+    // GOL_BATCH_SELECT_PLAN: the two plans' timed runs and which one was kept
+    double plan_tuned_ms = 0.0, plan_builtin_ms = 0.0;
+    bool plan_builtin = false;
     uint32_t rounds = 0;
     uint64_t items = 0;
     uint64_t cell_updates = 0;
@@ -351,13 +362,11 @@ int plan_queues(gol_batch* b, const std::map<LaunchKey, int>& lane_of, std::vect
     return GOL_OK;
 }
 
-// Plan shapes, rounds and items; allocate the arena and upload everything.
-int build(gol_batch* b) {
-    const int T = b->cfg.round_generations ? static_cast<int>(b->cfg.round_generations) : 32;
+// Allocate the arena (every grid's two buffers and its initial copy) and
+// upload the initial states; the host copies are released afterwards.
+int build_arena(gol_batch* b) {
     size_t off = 0;
     for (auto& g : b->grids) {
-        int rc = plan_batch_grid(g.x_dw, g.y_rows, T, true, &g.shape, &g.geom, &g.chunk);
-        if (rc) return rc;
         g.offset = off;
         const size_t n = static_cast<size_t>(g.x_dw * g.y_rows);
         off += 3 * ((n + 63) & ~size_t(63));  // buf0, buf1, initial copy; 256-byte aligned
@@ -365,8 +374,47 @@ int build(gol_batch* b) {
     b->arena_dwords = off;
     hipError_t e;
     if (off > 0 && (e = hipMalloc(&b->arena, off * 4)) != hipSuccess) return fail_hip(e, "batch arena");
+    for (auto& g : b->grids) {
+        const size_t n = static_cast<size_t>(g.x_dw * g.y_rows);
+        const size_t stride = (n + 63) & ~size_t(63);
+        if ((e = hipMemcpyAsync(b->arena + g.offset + 2 * stride, g.host.data(), n * 4, hipMemcpyHostToDevice,
+                                b->stream)) != hipSuccess ||
+            (e = hipMemcpyAsync(b->arena + g.offset, g.host.data(), n * 4, hipMemcpyHostToDevice, b->stream)) !=
+                hipSuccess)
+            return fail_hip(e, "batch upload");
+    }
+    if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return fail_hip(e, "batch upload");
+    for (auto& g : b->grids) {
+        g.host.clear();
+        g.host.shrink_to_fit();
+    }
+    return GOL_OK;
+}
 
-    // descriptors
+// Release the current plan: descriptors, items, queue state and graph.
+void free_plan(gol_batch* b) {
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->d_grids) (void)hipFree(b->d_grids);
+    if (b->d_items) (void)hipFree(b->d_items);
+    if (b->d_qstate) (void)hipFree(b->d_qstate);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+    b->d_grids = nullptr;
+    b->d_items = nullptr;
+    b->d_qstate = nullptr;
+    b->launches.clear();
+}
+
+// Plan shapes, rounds and items (the measured choices unless t_untuned is set)
+// and upload descriptors and items.
+int build_plan(gol_batch* b) {
+    const int T = b->cfg.round_generations ? static_cast<int>(b->cfg.round_generations) : 32;
+    for (auto& g : b->grids) {
+        int rc = plan_batch_grid(g.x_dw, g.y_rows, T, Planner::kBatch, &g.shape, &g.geom, &g.chunk);
+        if (rc) return rc;
+    }
+    hipError_t e;
     std::vector<BatchGrid> descs(b->grids.size());
     for (size_t i = 0; i < b->grids.size(); ++i) {
         auto& g = b->grids[i];
@@ -375,12 +423,6 @@ int build(gol_batch* b) {
         descs[i].buf[0] = b->arena + g.offset;
         descs[i].buf[1] = b->arena + g.offset + stride;
         descs[i].geo = to_device_geo(g.geom, g.x_dw, g.y_rows);
-        // initial copy and current state
-        if ((e = hipMemcpyAsync(b->arena + g.offset + 2 * stride, g.host.data(), n * 4, hipMemcpyHostToDevice,
-                                b->stream)) != hipSuccess ||
-            (e = hipMemcpyAsync(descs[i].buf[0], g.host.data(), n * 4, hipMemcpyHostToDevice, b->stream)) !=
-                hipSuccess)
-            return fail_hip(e, "batch upload");
     }
 
     // rounds of work items, grouped by shape, largest items first
@@ -476,11 +518,6 @@ int build(gol_batch* b) {
             return fail_hip(e, "batch queue state");
     }
     if ((e = hipStreamSynchronize(b->stream)) != hipSuccess) return fail_hip(e, "batch build");
-    for (auto& g : b->grids) {
-        g.host.clear();
-        g.host.shrink_to_fit();
-    }
-    b->built = true;
     return GOL_OK;
 }
 
@@ -529,6 +566,75 @@ int enqueue_rounds(gol_batch* b) {
     return GOL_OK;
 }
 
+// One run of the current plan: restore the initial states if a run changed
+// them, then every launch (replayed from the graph when use_graph).
+int run_once(gol_batch* b, double* gpu_ms) {
+    int rc;
+    if (b->dirty && (rc = gol_batch_reset(b))) return rc;
+    b->dirty = true;
+    hipError_t e = hipEventRecord(b->ev0, b->stream);
+    if (e != hipSuccess) return fail_hip(e, "hipEventRecord");
+    if (b->cfg.use_graph) {
+        if (!b->graph_exec) {
+            if ((e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+                return fail_hip(e, "hipStreamBeginCapture");
+            rc = enqueue_rounds(b);
+            hipGraph_t graph = nullptr;
+            e = hipStreamEndCapture(b->stream, &graph);
+            if (rc) return rc;
+            if (e != hipSuccess) return fail_hip(e, "hipStreamEndCapture");
+            b->graph = graph;
+            if ((e = hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0)) != hipSuccess)
+                return fail_hip(e, "hipGraphInstantiate");
+        }
+        if ((e = hipGraphLaunch(b->graph_exec, b->stream)) != hipSuccess) return fail_hip(e, "hipGraphLaunch");
+    } else if ((rc = enqueue_rounds(b))) {
+        return rc;
+    }
+    if ((e = hipEventRecord(b->ev1, b->stream)) != hipSuccess) return fail_hip(e, "hipEventRecord");
+    if ((e = hipEventSynchronize(b->ev1)) != hipSuccess) return fail_hip(e, "gol_batch_run");
+    if (b->queue && b->d_qstate) {
+        unsigned int err = 0;
+        if ((e = hipMemcpy(&err, b->d_qstate + b->qstate_words - 1, sizeof(err), hipMemcpyDeviceToHost)) !=
+            hipSuccess)
+            return fail_hip(e, "gol_batch_run (queue state)");
+        if (err)
+            return fail(GOL_ERR_RUNTIME, "persistent batch stopped: an item waited too long for its grid's earlier "
+                        "items (GOL_BATCH_PERSISTENT off selects one launch per round)");
+    }
+    if (gpu_ms) {
+        float ms = 0.f;
+        if ((e = hipEventElapsedTime(&ms, b->ev0, b->ev1)) != hipSuccess) return fail_hip(e, "hipEventElapsedTime");
+        *gpu_ms = ms;
+    }
+    b->runs += 1;
+    b->result_parity_valid = true;
+    for (auto& g : b->grids) g.result_parity = g.final_parity;
+    return GOL_OK;
+}
+
+// GOL_BATCH_SELECT_PLAN: time the batch under the measured per-class choices
+// and under the built-in model (one run each after a warm-up run that records
+// the graph) and keep the faster plan.
+int select_plan(gol_batch* b) {
+    int rc;
+    double tuned_ms = 0.0, builtin_ms = 0.0;
+    if ((rc = run_once(b, nullptr)) || (rc = run_once(b, &tuned_ms))) return rc;
+    free_plan(b);
+    t_untuned = true;
+    rc = build_plan(b);
+    t_untuned = false;
+    if (rc || (rc = run_once(b, nullptr)) || (rc = run_once(b, &builtin_ms))) return rc;
+    b->plan_tuned_ms = tuned_ms;
+    b->plan_builtin_ms = builtin_ms;
+    b->plan_builtin = builtin_ms <= tuned_ms;
+    if (!b->plan_builtin) {
+        free_plan(b);
+        if ((rc = build_plan(b))) return rc;
+    }
+    return GOL_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -543,7 +649,7 @@ int gol_batch_create(const gol_batch_config* config, gol_batch** out) {
     if (config) cfg = *config;
     if (cfg.round_generations > 32)
         return fail(GOL_ERR_INVALID_ARGUMENT, "round_generations must be at most 32 (the one-lane x halo)");
-    if (cfg.flags & ~(GOL_BATCH_PERSISTENT | GOL_BATCH_WAIT_PROBE))
+    if (cfg.flags & ~(GOL_BATCH_PERSISTENT | GOL_BATCH_WAIT_PROBE | GOL_BATCH_SELECT_PLAN))
         return fail(GOL_ERR_INVALID_ARGUMENT, "unknown batch flags 0x%x", cfg.flags);
     if ((cfg.flags & GOL_BATCH_WAIT_PROBE) && !(cfg.flags & GOL_BATCH_PERSISTENT))
         return fail(GOL_ERR_INVALID_ARGUMENT, "GOL_BATCH_WAIT_PROBE applies to persistent batches only");
@@ -609,6 +715,8 @@ int gol_batch_reset(gol_batch* b) {
                                       hipMemcpyDeviceToDevice, b->stream);
         if (r != hipSuccess) return fail_hip(r, "gol_batch_reset");
     }
+    b->dirty = false;
+    b->result_parity_valid = false;
     return GOL_OK;
 }
 
@@ -617,45 +725,13 @@ int gol_batch_run(gol_batch* b, double* gpu_ms) {
     if (!b) return fail(GOL_ERR_INVALID_ARGUMENT, "null batch");
     Guard g(b->device);
     int rc;
-    if (!b->built && (rc = build(b))) return rc;
-    if (b->runs > 0 && (rc = gol_batch_reset(b))) return rc;
-    hipError_t e = hipEventRecord(b->ev0, b->stream);
-    if (e != hipSuccess) return fail_hip(e, "hipEventRecord");
-    if (b->cfg.use_graph) {
-        if (!b->graph_exec) {
-            if ((e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
-                return fail_hip(e, "hipStreamBeginCapture");
-            rc = enqueue_rounds(b);
-            hipGraph_t graph = nullptr;
-            e = hipStreamEndCapture(b->stream, &graph);
-            if (rc) return rc;
-            if (e != hipSuccess) return fail_hip(e, "hipStreamEndCapture");
-            b->graph = graph;
-            if ((e = hipGraphInstantiate(&b->graph_exec, graph, nullptr, nullptr, 0)) != hipSuccess)
-                return fail_hip(e, "hipGraphInstantiate");
-        }
-        if ((e = hipGraphLaunch(b->graph_exec, b->stream)) != hipSuccess) return fail_hip(e, "hipGraphLaunch");
-    } else if ((rc = enqueue_rounds(b))) {
-        return rc;
+    if (!b->built) {
+        if ((rc = build_arena(b))) return rc;
+        b->built = true;  // from here on the arena holds the initial states
+        if ((rc = build_plan(b))) return rc;
+        if ((b->cfg.flags & GOL_BATCH_SELECT_PLAN) && (rc = select_plan(b))) return rc;
     }
-    if ((e = hipEventRecord(b->ev1, b->stream)) != hipSuccess) return fail_hip(e, "hipEventRecord");
-    if ((e = hipEventSynchronize(b->ev1)) != hipSuccess) return fail_hip(e, "gol_batch_run");
-    if (b->queue && b->d_qstate) {
-        unsigned int err = 0;
-        if ((e = hipMemcpy(&err, b->d_qstate + b->qstate_words - 1, sizeof(err), hipMemcpyDeviceToHost)) !=
-            hipSuccess)
-            return fail_hip(e, "gol_batch_run (queue state)");
-        if (err)
-            return fail(GOL_ERR_RUNTIME, "persistent batch stopped: an item waited too long for its grid's earlier "
-                        "items (GOL_BATCH_PERSISTENT off selects one launch per round)");
-    }
-    if (gpu_ms) {
-        float ms = 0.f;
-        if ((e = hipEventElapsedTime(&ms, b->ev0, b->ev1)) != hipSuccess) return fail_hip(e, "hipEventElapsedTime");
-        *gpu_ms = ms;
-    }
-    b->runs += 1;
-    return GOL_OK;
+    return run_once(b, gpu_ms);
 }
 
 int gol_batch_fetch(gol_batch* b, uint32_t grid_id, void* words, size_t n_words) {
@@ -666,13 +742,15 @@ int gol_batch_fetch(gol_batch* b, uint32_t grid_id, void* words, size_t n_words)
     const size_t n = static_cast<size_t>(g.x_dw * g.y_rows);
     if (n_words != n * 32 / g.word_bits)
         return fail(GOL_ERR_INVALID_ARGUMENT, "expected %zu words, got %zu", n * 32 / g.word_bits, n_words);
-    if (!b->built || b->runs == 0) {
+    if (!b->built) {
         std::memcpy(words, g.host.data(), n * 4);  // not run yet: the initial state
         return GOL_OK;
     }
     Guard gd(b->device);
     const size_t stride = (n + 63) & ~size_t(63);
-    const uint32_t* src = b->arena + g.offset + (g.final_parity ? stride : 0);
+    // the last completed run's result, else the initial state the arena keeps
+    const size_t at = b->result_parity_valid ? (g.result_parity ? stride : 0) : 2 * stride;
+    const uint32_t* src = b->arena + g.offset + at;
     hipError_t e = hipMemcpyAsync(words, src, n * 4, hipMemcpyDeviceToHost, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) return fail_hip(e, "gol_batch_fetch");
@@ -689,6 +767,9 @@ int gol_batch_get_info(gol_batch* b, gol_batch_info* info) {
     info->launches = b->launches.size() + (b->queue && b->built ? 1 : 0);  // + the queue reset
     info->device_bytes = b->arena_dwords * 4 + b->items * sizeof(WorkItem) + b->grids.size() * sizeof(BatchGrid);
     info->cell_updates = b->cell_updates;
+    info->plan_builtin = b->plan_builtin ? 1u : 0u;
+    info->plan_tuned_ms = b->plan_tuned_ms;
+    info->plan_builtin_ms = b->plan_builtin_ms;
     return GOL_OK;
 }
 
@@ -818,7 +899,7 @@ int gol_batch_autotune(const gol_tune_config* config, gol_tune_entry* entries, u
         TileGeometry base_geom;
         int base_round = 0;
         t_untuned = true;
-        rc = plan_batch_grid(x_dw, y_rows, R, true, &base_shape, &base_geom, &base_round);
+        rc = plan_batch_grid(x_dw, y_rows, R, Planner::kBatch, &base_shape, &base_geom, &base_round);
         t_untuned = false;
         if (rc) return rc;
 

@@ -20,11 +20,19 @@ TileShape default_shape(int64_t x_dw, int64_t y_rows);
 TileShape batch_default_shape(int64_t x_dw, int64_t y_rows);
 TileShape batch_shape(int64_t x_dw, int64_t y_rows);
 
-// Tile shape, geometry and generations per round of one grid in a batch or a
-// server: with use_tuned, the autotuner's choice for the size class when there
-// is one; otherwise grids split into tiles advance round_generations per
-// round, and grids that fit one tile advance a chunk of about the same cost.
-int plan_batch_grid(int64_t x_dw, int64_t y_rows, int round_generations, bool use_tuned, TileShape* shape,
+// Who plans a grid's rounds. A batch replays its rounds from a graph, so a
+// round costs little beyond its work: it takes the autotuner's measured choice
+// for the size class when there is one, and otherwise prices split grids with
+// loads and stores hidden behind other tiles' compute (which favours shallow
+// halos and many rounds). A server plans every round on the host and a job's
+// latency counts its rounds: it keeps the built-in choices and prices loads
+// and stores on top of compute, which favours deep halos and few rounds.
+enum class Planner { kBatch, kServer };
+
+// Tile shape, geometry and generations per round of one grid: grids split into
+// tiles advance up to round_generations per round, and grids that fit one tile
+// advance a chunk of about the same cost.
+int plan_batch_grid(int64_t x_dw, int64_t y_rows, int round_generations, Planner planner, TileShape* shape,
                     TileGeometry* geom, int* per_round);
 bool shape_compiled(const TileShape& s);
 

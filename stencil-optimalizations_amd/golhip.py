@@ -77,6 +77,7 @@ FLAG_SLAB_WAIT_PROBE = 2
 # gol_batch_config.flags
 BATCH_PERSISTENT = 1
 BATCH_WAIT_PROBE = 2
+BATCH_SELECT_PLAN = 4
 
 LAYOUT_ROWS, LAYOUT_COLUMNS, LAYOUT_TILES = 0, 1, 2
 
@@ -116,6 +117,10 @@ class GolBatchInfo(ctypes.Structure):
         ("launches", ctypes.c_uint64),
         ("device_bytes", ctypes.c_uint64),
         ("cell_updates", ctypes.c_uint64),
+        ("plan_builtin", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("plan_tuned_ms", ctypes.c_double),
+        ("plan_builtin_ms", ctypes.c_double),
     ]
 
 
@@ -338,6 +343,10 @@ class Batch:
         ms = ctypes.c_double()
         _check(self.lib.gol_batch_run(self.handle, ctypes.byref(ms)))
         return ms.value
+
+    def reset(self) -> None:
+        """Restore every grid's initial state (gol_batch_reset)."""
+        _check(self.lib.gol_batch_reset(self.handle))
 
     def fetch(self, gid: int) -> np.ndarray:
         x, y, wb = self.shapes[gid]

@@ -253,3 +253,33 @@ def test_persistent_batch_of_zero_generation_grids():
         b.run()
         for i, w in enumerate(words):
             assert np.array_equal(b.fetch(i), w)
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+def test_select_plan_keeps_results(persistent):
+    """GOL_BATCH_SELECT_PLAN times the batch under the measured choices and
+    under the built-in model and keeps the faster plan: both plans are timed,
+    and every grid is bit-exact whichever plan is kept; a reset restores the
+    initial states, which fetch then returns."""
+    specs = mixed_specs(24, 9, max_side=2048, max_gens=200)
+    try:
+        golhip.batch_autotune([(2048, 2048), (1024, 1024)], round_generations=16, generations=32,
+                              cells_per_measure=1 << 24)
+        with golhip.Batch(round_generations=16, persistent=persistent, flags=golhip.BATCH_SELECT_PLAN) as b:
+            words = []
+            for x, y, gens, p, seed in specs:
+                words.append(golhip.fill_bernoulli(x, y, seed, p))
+                b.add(words[-1], x, y, gens)
+            note("select plan, persistent", persistent)
+            b.run()
+            info = b.info()
+            assert info.plan_tuned_ms > 0 and info.plan_builtin_ms > 0
+            assert info.plan_builtin == (1 if info.plan_builtin_ms <= info.plan_tuned_ms else 0)
+            b.run()
+            for i, (x, y, gens, p, seed) in enumerate(specs):
+                assert np.array_equal(b.fetch(i), ol.bitpar_run(words[i], x, y, gens)), (x, y, gens)
+            b.reset()
+            for i in (0, 7):
+                assert np.array_equal(b.fetch(i), words[i])
+    finally:
+        golhip.batch_tuning_reset()
