@@ -193,10 +193,12 @@ int plan_batch_grid(int64_t x_dw, int64_t y_rows, int round_generations, Planner
     for (int i = 0; i < kNumTileShapes; ++i) {
         const TileShape& s = kTileShapes[i];
         const int occupancy = tile_occupancy(s);
-        for (int steps = round_generations; steps >= 1; --steps) {
+        // a server keeps rounds at least half as deep as round_generations
+        const int min_steps = planner == Planner::kServer ? std::max(1, round_generations / 2) : 1;
+        for (int steps = round_generations; steps >= min_steps; --steps) {
             TileGeometry g;
             if (!plan_tiles(s, x_dw, y_rows, steps, &g, &why)) continue;
-            const double c = split_cost_per_generation(s, g, steps, planner == Planner::kBatch ? occupancy : 1);
+            const double c = split_cost_per_generation(s, g, steps, occupancy);
             if (c < best) {
                 best = c;
                 *shape = s;

@@ -20,13 +20,14 @@ TileShape default_shape(int64_t x_dw, int64_t y_rows);
 TileShape batch_default_shape(int64_t x_dw, int64_t y_rows);
 TileShape batch_shape(int64_t x_dw, int64_t y_rows);
 
-// Who plans a grid's rounds. A batch replays its rounds from a graph, so a
-// round costs little beyond its work: it takes the autotuner's measured choice
-// for the size class when there is one, and otherwise prices split grids with
-// loads and stores hidden behind other tiles' compute (which favours shallow
-// halos and many rounds). A server plans every round on the host and a job's
-// latency counts its rounds: it keeps the built-in choices and prices loads
-// and stores on top of compute, which favours deep halos and few rounds.
+// Who plans a grid's rounds. Both price split grids with loads and stores
+// hidden behind other tiles' compute where a shape holds two or more
+// workgroups per CU, which favours shallow halos and many rounds. A batch
+// replays its rounds from a graph, so a round costs little beyond its work:
+// it takes the autotuner's measured choice for the size class when there is
+// one, and any halo depth. A server plans every round on the host and a job's
+// latency counts its rounds: it keeps the built-in choices and rounds at least
+// half as deep as round_generations.
 enum class Planner { kBatch, kServer };
 
 // Tile shape, geometry and generations per round of one grid: grids split into

@@ -523,8 +523,8 @@ int gol_server_submit(gol_server* s, uint64_t x_cells, uint64_t y_cells, uint32_
     j->generations = j->remaining = generations;
     j->dwords = static_cast<size_t>(j->x_dw * j->y_rows);
     j->bytes = 2 * ((j->dwords + 63) & ~size_t(63)) * 4;
-    // the built-in choices and deep rounds, not a batch's: those maximise
-    // throughput with shallower rounds, and a job's latency counts rounds
+    // the built-in choices and rounds at least half as deep as a batch may
+    // take: a job's latency counts its rounds
     int rc = plan_batch_grid(j->x_dw, j->y_rows, s->round_generations, Planner::kServer, &j->shape, &j->geom,
                              &j->chunk);
     if (rc) {
