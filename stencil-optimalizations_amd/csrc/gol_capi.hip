@@ -244,7 +244,7 @@ int plan(gol_handle* h) {
         // wavefronts loop over list slots: launch no more than the device
         // holds at once
         const int all = (h->n_tiles + kFineWaves - 1) / kFineWaves;
-        const int resident = fine_resident_workgroups(dw);
+        const int resident = fine_resident_workgroups(dw, steps, h->x_dw);
         h->fine.workgroups = resident > 0 ? std::min(all, resident) : all;
         h->shape = TileShape{dw, 1, kFineWaves};
         return GOL_OK;

@@ -296,24 +296,32 @@ __global__ __launch_bounds__(64 * NW) void tile_kernel(const uint32_t* __restric
 // Change tracking over small tiles. A wavefront advances one tile of DW
 // dwords (32*DW cells) x tile_rows rows by up to T generations: lane l holds
 // grid row y0 - T + l, so the wavefront spans the tile's rows and T halo rows
-// above and below (tile_rows = 64 - 2T), as DW + 2 dwords: the tile's dwords
-// plus one halo dword on each side (enough for T <= 32). Rows above and below
+// above and below (tile_rows = 64 - 2T), as the tile's DW dwords plus halo
+// dwords. For T <= 16 one halo dword serves both sides (the ring form, DW + 1
+// dwords): its bits 16..31 hold cells 16..31 of the dword west of the tile and
+// its bits 0..15 cells 0..15 of the dword east of it, each at its own bit, so
+// the lane's dwords form a ring (the halo dword is the west neighbour of the
+// tile's first dword and the east neighbour of its last) and every dword
+// takes the same two funnel shifts. The seam inside the halo dword (bits 15
+// and 16 see each other as neighbours) spoils one cell per generation from
+// 16 cells out, so the tile's cells stay exact for 16 generations. For
+// deeper blocks a whole halo dword lies on each side (DW + 2 dwords, exact
+// for 32 generations). Rows above and below
 // a lane's row come from the neighbouring lanes (DPP), a dword's west and east
 // neighbours from the lane's own dwords, so a wavefront needs no LDS and no
 // barrier, and the wavefronts of a workgroup take independent tiles. A halo
 // cell is wrong after g generations only if it lies within g cells of the
 // wavefront's edge; after T generations the tile's own cells are exact.
 // ---------------------------------------------------------------------------
-template <int DW, bool MASK>
-__device__ __forceinline__ void fine_generations(uint32_t (&s)[DW + 2], int gens, uint32_t rowmask,
-                                                 const uint32_t (&colmask)[DW + 2]) {
-    constexpr int N = DW + 2;
+template <int N, bool MASK, bool RING>
+__device__ __forceinline__ void fine_generations(uint32_t (&s)[N], int gens, uint32_t rowmask,
+                                                 const uint32_t (&colmask)[N]) {
     for (int g = 0; g < gens; ++g) {
         uint32_t h0[N], h1[N], e0[N], e1[N];
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-            const uint32_t w = west_of(j > 0 ? s[j - 1] : 0u, s[j]);
-            const uint32_t e = east_of(s[j], j + 1 < N ? s[j + 1] : 0u);
+            const uint32_t w = west_of(j > 0 ? s[j - 1] : (RING ? s[N - 1] : 0u), s[j]);
+            const uint32_t e = east_of(s[j], j + 1 < N ? s[j + 1] : (RING ? s[0] : 0u));
             const RowSum r = row_sum(w, s[j], e);
             h0[j] = r.h0;
             h1[j] = r.h1;
@@ -331,60 +339,86 @@ __device__ __forceinline__ void fine_generations(uint32_t (&s)[DW + 2], int gens
     }
 }
 
-// Advances one tile by gens generations. With tracking (tr.list_next set), a
-// tile whose cells changed appends the tiles of its 3 x 3 neighbourhood to
-// the next block's list: a tile joins once, when its mark goes from 0 to 1,
-// and the wavefront reserves its entries with one atomic add. The wavefront
-// clears the mark of a listed tile it advances (its list is done with it).
+// The words a lane reads for one tile: its row's DW tile dwords, then the
+// dwords west and east of the tile (clamped addresses where the tile meets
+// the grid's edge; the masks applied after the load clear what lies outside).
 template <int DW>
-__device__ __forceinline__ void fine_tile(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
-                                          const FineGeo& g, int gens, int tile, bool listed, const FineTrack& tr) {
-    constexpr int N = DW + 2;
+struct FineRow {
+    uint32_t v[DW + 2];
+};
+
+// Issues the lane's loads for `tile`: no branch around them, so all of them are
+// in flight together, and nothing waits for them here. VEC (rows of a
+// multiple of 4 dwords, so 16-byte aligned): 16-byte loads, each wholly inside
+// the row or, past its end, clamped to its start (x0 + 1 is a multiple of
+// DW >= 4); else one load per dword.
+template <int DW, bool VEC>
+__device__ __forceinline__ void fine_load(const uint32_t* __restrict__ src, const FineGeo& g, int tile,
+                                          FineRow<DW>& r) {
     const int lane = threadIdx.x & 63;
     const int tx = tile % g.tiles_x, ty = tile / g.tiles_x;
-    const int x0 = tx * DW - 1;                      // first dword held: the west halo dword
-    const int y0 = ty * g.tile_rows - g.halo_rows;   // row of lane 0
-    const int y = y0 + lane;
-    const bool row_in = y >= 0 && y < g.y_rows;
-    const uint32_t* row = src + static_cast<size_t>(row_in ? y : 0) * g.x_dw;
-
-    uint32_t s[N], colmask[N];
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        const int gx = x0 + j;
-        colmask[j] = (gx >= 0 && gx < g.x_dw) ? ~0u : 0u;
-    }
-    // the tile's own dwords: 16-byte loads when all DW lie in the row and rows
-    // start on 16-byte boundaries (x0 + 1 is a multiple of DW >= 4 dwords),
-    // else one clamped load per dword
-    const bool full_x = x0 + 1 + DW <= g.x_dw && (g.x_dw & 3) == 0;
-    if (full_x) {
-        const uint4* v = reinterpret_cast<const uint4*>(row + x0 + 1);
+    const int x0 = tx * DW - 1;
+    const int y = ty * g.tile_rows - g.halo_rows + lane;
+    const uint32_t* row = src + static_cast<size_t>(y >= 0 && y < g.y_rows ? y : 0) * g.x_dw;
+    if (VEC) {
 #pragma unroll
         for (int q = 0; q < DW / 4; ++q) {
-            const uint4 t = v[q];
-            s[1 + 4 * q] = t.x;
-            s[2 + 4 * q] = t.y;
-            s[3 + 4 * q] = t.z;
-            s[4 + 4 * q] = t.w;
+            const int c = x0 + 1 + 4 * q;
+            const uint4 w = *reinterpret_cast<const uint4*>(row + (c < g.x_dw ? c : 0));
+            r.v[4 * q] = w.x;
+            r.v[4 * q + 1] = w.y;
+            r.v[4 * q + 2] = w.z;
+            r.v[4 * q + 3] = w.w;
         }
     } else {
 #pragma unroll
-        for (int j = 1; j <= DW; ++j) s[j] = row[colmask[j] ? x0 + j : 0] & colmask[j];
+        for (int d = 0; d < DW; ++d) r.v[d] = row[x0 + 1 + d < g.x_dw ? x0 + 1 + d : 0];
     }
-    s[0] = row[colmask[0] ? x0 : 0] & colmask[0];
-    s[N - 1] = row[colmask[N - 1] ? x0 + N - 1 : 0] & colmask[N - 1];
+    r.v[DW] = row[x0 >= 0 ? x0 : 0];
+    r.v[DW + 1] = row[x0 + DW + 1 < g.x_dw ? x0 + DW + 1 : 0];
+}
+
+// Advances one tile, whose words fine_load read, by gens generations. With
+// tracking (tr.list_next set), a tile whose cells changed appends the tiles of
+// its 3 x 3 neighbourhood to the next block's list: a tile joins once, when
+// its mark goes from 0 to 1, and the wavefront reserves its entries with one
+// atomic add. The wavefront clears the mark of a listed tile it advances (its
+// list is done with it).
+template <int DW, bool RING, bool VEC>
+__device__ __forceinline__ void fine_tile(const FineRow<DW>& r, uint32_t* __restrict__ dst, const FineGeo& g,
+                                          int gens, int tile, bool listed, const FineTrack& tr) {
+    constexpr int N = RING ? DW + 1 : DW + 2;  // s[0]: (west) halo, s[1..DW]: the tile, s[DW + 1]: east halo
+    const int lane = threadIdx.x & 63;
+    const int tx = tile % g.tiles_x, ty = tile / g.tiles_x;
+    const int x0 = tx * DW - 1;                      // the dword west of the tile
+    const int y0 = ty * g.tile_rows - g.halo_rows;   // row of lane 0
+    const int y = y0 + lane;
+    const bool row_in = y >= 0 && y < g.y_rows;
+    const bool west_in = x0 >= 0, east_in = x0 + DW + 1 < g.x_dw;  // the halo dwords lie in the grid
     const uint32_t rowmask = row_in ? ~0u : 0u;
+
+    uint32_t s[N], colmask[N];
 #pragma unroll
-    for (int j = 0; j < N; ++j) s[j] &= rowmask;
+    for (int j = 1; j <= DW; ++j) colmask[j] = x0 + j < g.x_dw ? ~0u : 0u;
+    if (RING) {
+        colmask[0] = (west_in ? 0xffff0000u : 0u) | (east_in ? 0x0000ffffu : 0u);
+        s[0] = ((r.v[DW] & 0xffff0000u) | (r.v[DW + 1] & 0x0000ffffu)) & colmask[0] & rowmask;
+    } else {
+        colmask[0] = west_in ? ~0u : 0u;
+        colmask[N - 1] = east_in ? ~0u : 0u;
+        s[0] = r.v[DW] & colmask[0] & rowmask;
+        s[N - 1] = r.v[DW + 1] & colmask[N - 1] & rowmask;
+    }
+#pragma unroll
+    for (int d = 0; d < DW; ++d) s[1 + d] = r.v[d] & colmask[1 + d] & rowmask;
 
     uint32_t before[DW];
 #pragma unroll
     for (int d = 0; d < DW; ++d) before[d] = s[1 + d];
 
-    const bool edge = y0 < 0 || y0 + 64 > g.y_rows || x0 < 0 || x0 + N > g.x_dw;  // wave-uniform
-    if (edge) fine_generations<DW, true>(s, gens, rowmask, colmask);
-    else fine_generations<DW, false>(s, gens, rowmask, colmask);
+    const bool edge = y0 < 0 || y0 + 64 > g.y_rows || !west_in || !east_in;  // wave-uniform
+    if (edge) fine_generations<N, true, RING>(s, gens, rowmask, colmask);
+    else fine_generations<N, false, RING>(s, gens, rowmask, colmask);
 
     const bool own = lane >= g.halo_rows && lane < g.halo_rows + g.tile_rows && row_in;
     bool diff = false;
@@ -392,11 +426,12 @@ __device__ __forceinline__ void fine_tile(const uint32_t* __restrict__ src, uint
     for (int d = 0; d < DW; ++d) diff |= before[d] != s[1 + d];
     if (own) {
         uint32_t* out = dst + static_cast<size_t>(y) * g.x_dw;
-        if (full_x) {
-            uint4* v = reinterpret_cast<uint4*>(out + x0 + 1);
+        if (VEC) {
 #pragma unroll
             for (int q = 0; q < DW / 4; ++q)
-                v[q] = make_uint4(s[1 + 4 * q], s[2 + 4 * q], s[3 + 4 * q], s[4 + 4 * q]);
+                if (colmask[1 + 4 * q])
+                    *reinterpret_cast<uint4*>(out + x0 + 1 + 4 * q) =
+                        make_uint4(s[1 + 4 * q], s[2 + 4 * q], s[3 + 4 * q], s[4 + 4 * q]);
         } else {
 #pragma unroll
             for (int j = 1; j <= DW; ++j)
@@ -428,10 +463,12 @@ __device__ __forceinline__ void fine_tile(const uint32_t* __restrict__ src, uint
 }
 
 // Wavefronts take slots i, i + stride, ... of tile list[0 .. *count), or of
-// tiles 0 .. dense_tiles - 1 when list is null (the grid is at most what the
-// device holds at once, so few wavefronts find no slot). The launch resets
-// the counter of the list after next.
-template <int DW>
+// tiles 0 .. dense_tiles - 1 when list is null, and read one tile ahead: the
+// loads of the next tile and the list entry after it are in flight while the
+// current tile advances (a block lists more tiles than the launch has
+// wavefronts when much of the grid is active). The launch resets the counter
+// of the list after next.
+template <int DW, bool RING, bool VEC>
 __global__ __launch_bounds__(64 * kFineWaves) void fine_tile_kernel(const uint32_t* __restrict__ src,
                                                                     uint32_t* __restrict__ dst, FineGeo g, int gens,
                                                                     const int* __restrict__ list,
@@ -440,10 +477,34 @@ __global__ __launch_bounds__(64 * kFineWaves) void fine_tile_kernel(const uint32
     if (tr.count_clear && blockIdx.x == 0 && threadIdx.x == 0) *tr.count_clear = 0;
     const int n = list ? *count : dense_tiles;
     const int stride = static_cast<int>(gridDim.x) * kFineWaves;
-    for (int slot = blockIdx.x * kFineWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); slot < n;
-         slot += stride) {
-        const int tile = list ? __builtin_amdgcn_readfirstlane(list[slot]) : slot;
-        fine_tile<DW>(src, dst, g, gens, tile, list != nullptr, tr);
+    int slot = blockIdx.x * kFineWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (slot >= n) return;
+    auto tile_at = [&](int s) { return list ? __builtin_amdgcn_readfirstlane(list[s]) : s; };
+    int tile = tile_at(slot);
+    int next_tile = slot + stride < n ? tile_at(slot + stride) : -1;
+    FineRow<DW> cur;
+    fine_load<DW, VEC>(src, g, tile, cur);
+    for (;;) {
+        // the words arrive here, not earlier: nothing the compiler would hoist
+        // to the loads that fetch them makes the wavefront wait for them there
+#pragma unroll
+        for (int i = 0; i < DW + 2; ++i) asm volatile("" : "+v"(cur.v[i]));
+        FineRow<DW> nxt;
+        int after = -1;
+        if (next_tile >= 0) {
+            fine_load<DW, VEC>(src, g, next_tile, nxt);
+            if (slot + 2 * stride < n) after = tile_at(slot + 2 * stride);
+        }
+        // keeps the loads here, issued before the tile advances (the compiler
+        // would otherwise sink them to their use); it reads no loaded word, so
+        // nothing waits for them
+        asm volatile("" ::: "memory");
+        fine_tile<DW, RING, VEC>(cur, dst, g, gens, tile, list != nullptr, tr);
+        if (next_tile < 0) return;
+        slot += stride;
+        tile = next_tile;
+        next_tile = after;
+        cur = nxt;
     }
 }
 
@@ -955,30 +1016,42 @@ int tile_occupancy(const TileShape& shape) {
     return per_cu;
 }
 
-static const void* fine_function(int dw) {
-    return dw == 4 ? reinterpret_cast<const void*>(&fine_tile_kernel<4>)
-                   : reinterpret_cast<const void*>(&fine_tile_kernel<8>);
+// The change-tracking kernel for tiles of dw dwords, the ring form or not,
+// 16-byte loads or not.
+template <int DW>
+static auto fine_kernel_for(bool ring, bool vec) {
+    using K = decltype(&fine_tile_kernel<DW, true, true>);
+    if (ring) return vec ? K(&fine_tile_kernel<DW, true, true>) : K(&fine_tile_kernel<DW, true, false>);
+    return vec ? K(&fine_tile_kernel<DW, false, true>) : K(&fine_tile_kernel<DW, false, false>);
 }
+
+static bool fine_ring(int halo_rows) { return halo_rows <= kFineRingMaxSteps; }
+static bool fine_vec(int64_t x_dw) { return (x_dw & 3) == 0; }
 
 hipError_t launch_fine_tiles(const FineGeo& g, const uint32_t* src, uint32_t* dst, int gens, const int* list,
                              const int* count, const FineTrack& track, hipStream_t stream) {
+    if ((g.tile_dwords != 4 && g.tile_dwords != 8) || gens > g.halo_rows || g.halo_rows > kFineMaxSteps)
+        return hipErrorInvalidValue;
     const int n = g.tiles_x * g.tiles_y;
     const dim3 grid(static_cast<unsigned>(fine_workgroups(g))), block(64 * kFineWaves);
+    const bool ring = fine_ring(g.halo_rows), vec = fine_vec(g.x_dw);
     if (g.tile_dwords == 4)
-        hipLaunchKernelGGL(fine_tile_kernel<4>, grid, block, 0, stream, src, dst, g, gens, list, count, n, track);
-    else if (g.tile_dwords == 8)
-        hipLaunchKernelGGL(fine_tile_kernel<8>, grid, block, 0, stream, src, dst, g, gens, list, count, n, track);
+        hipLaunchKernelGGL(fine_kernel_for<4>(ring, vec), grid, block, 0, stream, src, dst, g, gens, list, count, n,
+                           track);
     else
-        return hipErrorInvalidValue;
+        hipLaunchKernelGGL(fine_kernel_for<8>(ring, vec), grid, block, 0, stream, src, dst, g, gens, list, count, n,
+                           track);
     return hipGetLastError();
 }
 
-int fine_resident_workgroups(int tile_dwords) {
+int fine_resident_workgroups(int tile_dwords, int halo_rows, int64_t x_dw) {
+    const bool ring = fine_ring(halo_rows), vec = fine_vec(x_dw);
+    const void* f = tile_dwords == 4 ? reinterpret_cast<const void*>(fine_kernel_for<4>(ring, vec))
+                                     : reinterpret_cast<const void*>(fine_kernel_for<8>(ring, vec));
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fine_function(tile_dwords), 64 * kFineWaves, 0) !=
-            hipSuccess)
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 64 * kFineWaves, 0) != hipSuccess)
         return 0;
     return cus * per_cu;
 }

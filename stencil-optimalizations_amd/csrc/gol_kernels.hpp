@@ -100,6 +100,10 @@ int tile_occupancy(const TileShape& shape);
 constexpr int kFineWaves = 4;      // wavefronts (tiles) per workgroup
 constexpr int kFineDefaultDW = 4;  // tile width in dwords when gol_config.lane_dwords is 0
 constexpr int kFineMaxSteps = 21;  // T <= tile_rows: a tile is at least as tall as T generations reach
+// Blocks of T <= 16 generations need at most 16 halo cells on each side of a
+// tile: one dword holds both (the ring form, DW + 1 dwords per lane); deeper
+// blocks keep a whole halo dword on each side (DW + 2).
+constexpr int kFineRingMaxSteps = 16;
 
 struct FineGeo {
     int x_dw, y_rows;
@@ -113,9 +117,10 @@ struct FineGeo {
 // Workgroups of a change-tracking launch (kFineWaves wavefronts each).
 inline int fine_workgroups(const FineGeo& g) { return g.workgroups; }
 
-// Workgroups the change-tracking kernel for tiles of tile_dwords can hold
-// resident at once on the current device (0 if unknown).
-int fine_resident_workgroups(int tile_dwords);
+// Workgroups the change-tracking kernel for tiles of tile_dwords, blocks of
+// halo_rows generations and rows of x_dw dwords can hold resident at once on
+// the current device (0 if unknown).
+int fine_resident_workgroups(int tile_dwords, int halo_rows, int64_t x_dw);
 
 // Change-tracking state of one block (all null: no tracking).
 struct FineTrack {

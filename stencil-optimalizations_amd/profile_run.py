@@ -7,6 +7,8 @@
   python3 profile_run.py plain    16384^2 random, 100 generations with the plain kernel
   python3 profile_run.py active   16384^2 mostly-stable patches, 20000 generations, change tracking
                                   (the bench's mostly-stable configuration)
+  python3 profile_run.py spacefiller  16384^2 empty with spacefiller[8192, 8192], 20000 generations,
+                                  change tracking (the bench's spacefiller configuration)
   python3 profile_run.py batch    the bench's batched workload: 4096 uneven grids (sides 64..2048,
                                   100..5000 generations), built once, then two graph replays
   python3 profile_run.py batch_queue  the same batch as persistent queue launches (one per kernel
@@ -52,13 +54,16 @@ def main(what: str) -> None:
                 print(f"slab: 10000 generations, {ms:.2f} ms GPU", flush=True)
             info = g.info()
         print(f"slab: {info.launches} launches", flush=True)
-    elif what == "active":
-        w = golhip.fill_patches(16384, 16384, 42, 256, 64, 0.35)
+    elif what in ("active", "spacefiller"):
+        if what == "active":
+            w = golhip.fill_patches(16384, 16384, 42, 256, 64, 0.35)
+        else:
+            w = golhip.fill_lexicon(16384, 16384, "spacefiller[8192, 8192]")
         with golhip.Grid(16384, 16384, kernel=golhip.KERNEL_ACTIVE, device=0) as g:
             g.upload(w)
             ms = g.step_timed(20000)
             info = g.info()
-        print(f"active: 20000 generations, {info.launches} launches, {ms:.2f} ms GPU, "
+        print(f"{what}: 20000 generations, {info.launches} launches, {ms:.2f} ms GPU, "
               f"{info.active_tiles_total} active tile-blocks of {info.tiles * info.tracked_blocks}", flush=True)
     elif what in ("batch", "batch_queue"):
         specs = workloads.batched_uneven(4096)
