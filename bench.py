@@ -454,15 +454,18 @@ def main() -> int:
     bytes_per_launch = total_updates * BYTES_PER_CELL_UPDATE / launches
     achieved_gbs = bytes_per_launch / avg_launch_s / 1e9
     # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes
-    # (calibrated against a copy kernel), valid for the profiled shape and depth
+    # (calibrated against a copy kernel), valid for the profiled shape and depth only
     traffic = None
     prof = os.path.join(ROOT, "profiles", "slab_kernel_16384x16384_random.md" if info.persistent
                         else "tile_kernel_16384x16384_random.md")
-    if (os.path.exists(prof) and (info.lane_dwords, info.band_rows, info.waves, info.temporal_steps) == (8, 10, 8, 8)
-            and args.kernel == "temporal"):
+    if os.path.exists(prof) and args.kernel == "temporal":
         import re
-        m = re.search(r"hbm_bytes_per_launch:\s*(\d+)", open(prof).read())
-        traffic = int(m.group(1)) if m else None
+        text = open(prof).read()
+        m = re.search(r"hbm_bytes_per_launch:\s*(\d+)", text)
+        s = re.search(r"profiled_shape:\s*(\d+)\s+(\d+)\s+(\d+)\s+(\d+)", text)
+        if m and s and tuple(int(v) for v in s.groups()) == (info.lane_dwords, info.band_rows, info.waves,
+                                                                 info.temporal_steps):
+            traffic = int(m.group(1))
 
     result = {
         "metric": METRIC,

@@ -2,8 +2,9 @@
 
   python3 profile_run.py slab     16384^2 random, 3 persistent slab launches of 10000 generations
                                   (the bench's configuration: one launch per step)
-  python3 profile_run.py tile     16384^2 random, 800 generations with the temporal-tile kernel
-                                  (100 launches of 8 generations, persistent launch switched off)
+  python3 profile_run.py tile     16384^2 random, 2400 generations with the temporal-tile kernel at
+                                  the shape and depth the engine chooses with the persistent launch
+                                  switched off (tile_kernel<1, 32, 8>, 100 launches of 24 generations)
   python3 profile_run.py plain    16384^2 random, 100 generations with the plain kernel
   python3 profile_run.py active   16384^2 mostly-stable patches, 20000 generations, change tracking
                                   (the bench's mostly-stable configuration)
@@ -37,12 +38,14 @@ def main(what: str) -> None:
     if what in ("tile", "plain"):
         w = golhip.fill_random(16384, 16384, 42)
         kernel = golhip.KERNEL_TEMPORAL if what == "tile" else golhip.KERNEL_PLAIN
-        gens = 800 if what == "tile" else 100
+        gens = 2400 if what == "tile" else 100
         with golhip.Grid(16384, 16384, kernel=kernel, device=0, flags=golhip.FLAG_NO_PERSISTENT) as g:
             g.upload(w)
             ms = g.step_timed(gens)
             info = g.info()
-        print(f"{what}: {gens} generations, {info.launches} launches, {ms:.2f} ms GPU", flush=True)
+        print(f"{what}: {gens} generations, {info.launches} launches, {ms:.2f} ms GPU, shape "
+              f"({info.lane_dwords}, {info.band_rows}, {info.waves}), {info.temporal_steps} generations per launch",
+              flush=True)
     elif what == "slab":
         w = golhip.fill_random(16384, 16384, 42)
         with golhip.Grid(16384, 16384, kernel=golhip.KERNEL_TEMPORAL, device=0) as g:
