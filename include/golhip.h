@@ -60,7 +60,7 @@ enum gol_kernel {
                                 32 * lane_dwords cells (lane_dwords 4 or 8, default 4) x
                                 (64 - 2 * temporal_steps) rows is advanced only when it or a
                                 neighbour changed over the last block of temporal_steps (1..21,
-                                default 12) generations; band_rows and waves do not apply
+                                default 16) generations; band_rows and waves do not apply
                                 (reference: cuda-naive-local-one-cell/cuda_local_one_cell.cu:53-242) */
 };
 

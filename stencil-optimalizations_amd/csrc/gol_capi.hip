@@ -221,11 +221,13 @@ int plan(gol_handle* h) {
     if (h->kernel == GOL_KERNEL_ACTIVE) {
         // Change tracking skips quiet regions tile by tile, so its tiles are
         // small: 32 * lane_dwords cells (lane_dwords 4 or 8) x (64 - 2T)
-        // rows, one wavefront each. 12 generations per block by default:
-        // oscillators of period 1, 2, 3, 4, 6 and 12 return to their state at
-        // block boundaries and count as unchanged. band_rows and waves do not
-        // apply.
-        const int steps = c.temporal_steps ? static_cast<int>(c.temporal_steps) : 12;
+        // rows, one wavefront each. kFineRingMaxSteps (16) generations per
+        // block by default, the deepest block one shared halo dword covers:
+        // oscillators of period 1, 2, 4, 8 and 16 return to their state at
+        // block boundaries and count as unchanged; those of period 3 (pulsars)
+        // keep their neighbourhood listed, which 12 would avoid at the cost of
+        // taller halos and more blocks. band_rows and waves do not apply.
+        const int steps = c.temporal_steps ? static_cast<int>(c.temporal_steps) : kFineRingMaxSteps;
         if (steps < 1 || steps > kFineMaxSteps)
             return fail(GOL_ERR_INVALID_ARGUMENT, "temporal_steps %d: change tracking takes 1 to %d generations per block",
                         steps, kFineMaxSteps);

@@ -109,6 +109,49 @@ def test_oscillators_stay_correct_between_blocks():
     assert info.active_tiles_total == info.tiles * info.tracked_blocks
 
 
+PULSAR = [
+    "..OOO...OOO..",
+    ".............",
+    "O....O.O....O",
+    "O....O.O....O",
+    "O....O.O....O",
+    "..OOO...OOO..",
+    ".............",
+    "..OOO...OOO..",
+    "O....O.O....O",
+    "O....O.O....O",
+    "O....O.O....O",
+    ".............",
+    "..OOO...OOO..",
+]
+
+
+def test_pulsars_quiet_at_12_listed_at_16():
+    # period-3 oscillators return to their state at boundaries of 12-generation
+    # blocks but not of 16-generation ones (the default): there they keep their
+    # tiles listed; the result is exact either way
+    x, y = 2048, 512
+    c = np.zeros((y, x), np.uint8)
+    p = np.array([[ch == "O" for ch in row] for row in PULSAR], np.uint8)
+    n = 0
+    for yy in range(40, y - 40, 96):
+        for xx in range(40, x - 40, 160):
+            c[yy:yy + 13, xx:xx + 13] = p
+            n += 1
+    gens = 16 * 12
+    want = ol.naive_run(c, gens)
+    note("active pulsars", n)
+    out12, info12 = run(ol.pack(c), x, y, gens, temporal_steps=12)
+    out16, info16 = run(ol.pack(c), x, y, gens)
+    assert info16.temporal_steps == 16
+    assert np.array_equal(ol.unpack(out12, x, y), want)
+    assert np.array_equal(ol.unpack(out16, x, y), want)
+    # 12: every tile in the first block, then nothing changes at a boundary
+    assert info12.active_tiles_total == info12.tiles
+    # 16: each pulsar keeps at least its own tile listed in every block
+    assert info16.active_tiles_total >= info16.tiles + n * (info16.tracked_blocks - 1)
+
+
 def test_split_calls_and_reupload():
     x, y = 2048, 600
     w = golhip.fill_patches(x, y, 11, 6, 64, 0.35)
