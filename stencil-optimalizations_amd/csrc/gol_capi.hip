@@ -220,7 +220,7 @@ int plan(gol_handle* h) {
     }
     if (h->kernel == GOL_KERNEL_ACTIVE) {
         // Change tracking skips quiet regions tile by tile, so its tiles are
-        // small: 32 * lane_dwords cells (lane_dwords 4 or 8) x (64 - 2T)
+        // small: 32 * lane_dwords cells (lane_dwords 2, 4 or 8) x (64 - 2T)
         // rows, one wavefront each. kFineRingMaxSteps (16) generations per
         // block by default, the deepest block one shared halo dword covers:
         // oscillators of period 1, 2, 4, 8 and 16 return to their state at
@@ -232,8 +232,9 @@ int plan(gol_handle* h) {
             return fail(GOL_ERR_INVALID_ARGUMENT, "temporal_steps %d: change tracking takes 1 to %d generations per block",
                         steps, kFineMaxSteps);
         const int dw = c.lane_dwords ? static_cast<int>(c.lane_dwords) : kFineDefaultDW;
-        if (dw != 4 && dw != 8)
-            return fail(GOL_ERR_INVALID_ARGUMENT, "lane_dwords %d: change-tracking tiles are 4 or 8 dwords wide", dw);
+        if (dw != 2 && dw != 4 && dw != 8)
+            return fail(GOL_ERR_INVALID_ARGUMENT, "lane_dwords %d: change-tracking tiles are 2, 4 or 8 dwords wide",
+                        dw);
         h->steps = steps;
         h->fine.x_dw = static_cast<int>(h->x_dw);
         h->fine.y_rows = static_cast<int>(h->y_rows);

@@ -349,12 +349,17 @@ struct FineRow {
 
 // Issues the lane's loads for `tile`: no branch around them, so all of them are
 // in flight together, and nothing waits for them here. VEC (rows of a
-// multiple of 4 dwords, so 16-byte aligned): 16-byte loads, each wholly inside
-// the row or, past its end, clamped to its start (x0 + 1 is a multiple of
-// DW >= 4); else one load per dword.
+// multiple of kFineChunk<DW> dwords, so aligned to one chunk): 16-byte loads
+// (8-byte for 2-dword tiles), each wholly inside the row or, past its end,
+// clamped to its start (x0 + 1 is a multiple of the chunk); else one load
+// per dword.
+template <int DW>
+constexpr int kFineChunk = DW >= 4 ? 4 : DW;
+
 template <int DW, bool VEC>
 __device__ __forceinline__ void fine_load(const uint32_t* __restrict__ src, const FineGeo& g, int tile,
                                           FineRow<DW>& r) {
+    constexpr int C = kFineChunk<DW>;
     const int lane = threadIdx.x & 63;
     const int tx = tile % g.tiles_x, ty = tile / g.tiles_x;
     const int x0 = tx * DW - 1;
@@ -362,13 +367,20 @@ __device__ __forceinline__ void fine_load(const uint32_t* __restrict__ src, cons
     const uint32_t* row = src + static_cast<size_t>(y >= 0 && y < g.y_rows ? y : 0) * g.x_dw;
     if (VEC) {
 #pragma unroll
-        for (int q = 0; q < DW / 4; ++q) {
-            const int c = x0 + 1 + 4 * q;
-            const uint4 w = *reinterpret_cast<const uint4*>(row + (c < g.x_dw ? c : 0));
-            r.v[4 * q] = w.x;
-            r.v[4 * q + 1] = w.y;
-            r.v[4 * q + 2] = w.z;
-            r.v[4 * q + 3] = w.w;
+        for (int q = 0; q < DW / C; ++q) {
+            const int c = x0 + 1 + C * q;
+            const uint32_t* at = row + (c < g.x_dw ? c : 0);
+            if constexpr (C == 4) {
+                const uint4 w = *reinterpret_cast<const uint4*>(at);
+                r.v[4 * q] = w.x;
+                r.v[4 * q + 1] = w.y;
+                r.v[4 * q + 2] = w.z;
+                r.v[4 * q + 3] = w.w;
+            } else {
+                const uint2 w = *reinterpret_cast<const uint2*>(at);
+                r.v[2 * q] = w.x;
+                r.v[2 * q + 1] = w.y;
+            }
         }
     } else {
 #pragma unroll
@@ -427,11 +439,16 @@ __device__ __forceinline__ void fine_tile(const FineRow<DW>& r, uint32_t* __rest
     if (own) {
         uint32_t* out = dst + static_cast<size_t>(y) * g.x_dw;
         if (VEC) {
+            constexpr int C = kFineChunk<DW>;
 #pragma unroll
-            for (int q = 0; q < DW / 4; ++q)
-                if (colmask[1 + 4 * q])
+            for (int q = 0; q < DW / C; ++q) {
+                if (!colmask[1 + C * q]) continue;
+                if constexpr (C == 4)
                     *reinterpret_cast<uint4*>(out + x0 + 1 + 4 * q) =
                         make_uint4(s[1 + 4 * q], s[2 + 4 * q], s[3 + 4 * q], s[4 + 4 * q]);
+                else
+                    *reinterpret_cast<uint2*>(out + x0 + 1 + 2 * q) = make_uint2(s[1 + 2 * q], s[2 + 2 * q]);
+            }
         } else {
 #pragma unroll
             for (int j = 1; j <= DW; ++j)
@@ -1017,7 +1034,7 @@ int tile_occupancy(const TileShape& shape) {
 }
 
 // The change-tracking kernel for tiles of dw dwords, the ring form or not,
-// 16-byte loads or not.
+// vector loads or not.
 template <int DW>
 static auto fine_kernel_for(bool ring, bool vec) {
     using K = decltype(&fine_tile_kernel<DW, true, true>);
@@ -1026,16 +1043,20 @@ static auto fine_kernel_for(bool ring, bool vec) {
 }
 
 static bool fine_ring(int halo_rows) { return halo_rows <= kFineRingMaxSteps; }
-static bool fine_vec(int64_t x_dw) { return (x_dw & 3) == 0; }
+static bool fine_vec(int64_t x_dw, int tile_dwords) { return x_dw % (tile_dwords >= 4 ? 4 : tile_dwords) == 0; }
 
 hipError_t launch_fine_tiles(const FineGeo& g, const uint32_t* src, uint32_t* dst, int gens, const int* list,
                              const int* count, const FineTrack& track, hipStream_t stream) {
-    if ((g.tile_dwords != 4 && g.tile_dwords != 8) || gens > g.halo_rows || g.halo_rows > kFineMaxSteps)
+    if ((g.tile_dwords != 2 && g.tile_dwords != 4 && g.tile_dwords != 8) || gens > g.halo_rows ||
+        g.halo_rows > kFineMaxSteps)
         return hipErrorInvalidValue;
     const int n = g.tiles_x * g.tiles_y;
     const dim3 grid(static_cast<unsigned>(fine_workgroups(g))), block(64 * kFineWaves);
-    const bool ring = fine_ring(g.halo_rows), vec = fine_vec(g.x_dw);
-    if (g.tile_dwords == 4)
+    const bool ring = fine_ring(g.halo_rows), vec = fine_vec(g.x_dw, g.tile_dwords);
+    if (g.tile_dwords == 2)
+        hipLaunchKernelGGL(fine_kernel_for<2>(ring, vec), grid, block, 0, stream, src, dst, g, gens, list, count, n,
+                           track);
+    else if (g.tile_dwords == 4)
         hipLaunchKernelGGL(fine_kernel_for<4>(ring, vec), grid, block, 0, stream, src, dst, g, gens, list, count, n,
                            track);
     else
@@ -1045,9 +1066,10 @@ hipError_t launch_fine_tiles(const FineGeo& g, const uint32_t* src, uint32_t* ds
 }
 
 int fine_resident_workgroups(int tile_dwords, int halo_rows, int64_t x_dw) {
-    const bool ring = fine_ring(halo_rows), vec = fine_vec(x_dw);
-    const void* f = tile_dwords == 4 ? reinterpret_cast<const void*>(fine_kernel_for<4>(ring, vec))
-                                     : reinterpret_cast<const void*>(fine_kernel_for<8>(ring, vec));
+    const bool ring = fine_ring(halo_rows), vec = fine_vec(x_dw, tile_dwords);
+    const void* f = tile_dwords == 2   ? reinterpret_cast<const void*>(fine_kernel_for<2>(ring, vec))
+                    : tile_dwords == 4 ? reinterpret_cast<const void*>(fine_kernel_for<4>(ring, vec))
+                                       : reinterpret_cast<const void*>(fine_kernel_for<8>(ring, vec));
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||

@@ -94,7 +94,7 @@ TileGeo to_device_geo(const TileGeometry& g, int64_t x_dw, int64_t y_rows);
 int tile_occupancy(const TileShape& shape);
 
 // Change tracking over small tiles: one wavefront per tile of tile_dwords
-// (4 or 8: 128 or 256 cells) x tile_rows rows, lane l holding row y0 + l of
+// (2, 4 or 8: 64, 128 or 256 cells) x tile_rows rows, lane l holding row y0 + l of
 // the tile's rows and halo_rows (= T) halo rows above and below:
 // tile_rows = 64 - 2T.
 constexpr int kFineWaves = 4;      // wavefronts (tiles) per workgroup
@@ -107,7 +107,7 @@ constexpr int kFineRingMaxSteps = 16;
 
 struct FineGeo {
     int x_dw, y_rows;
-    int tile_dwords;  // 4 or 8
+    int tile_dwords;  // 2, 4 or 8
     int tiles_x, tiles_y;
     int tile_rows;    // 64 - 2 * halo_rows
     int halo_rows;    // T: generations per block

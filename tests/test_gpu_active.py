@@ -42,13 +42,15 @@ def test_patches_against_naive(steps):
 
 # Tiles are 32 * lane_dwords cells x (64 - 2T) rows: partial last tile
 # columns and rows, odd dword counts (rows-32), grids shorter than a
-# wavefront, T = 1 and 21, both tile widths, and both halo forms: one halo
+# wavefront, T = 1 and 21, every tile width, and both halo forms: one halo
 # dword shared by both sides up to T = 16, a dword on each side from T = 17.
 @pytest.mark.parametrize("x,y,steps,word_bits,dw", [
     (32, 1, 12, 32, 8), (96, 5, 3, 32, 8), (2112, 70, 12, 64, 8), (288, 131, 21, 32, 8), (4128, 300, 1, 32, 8),
     (640, 64, 20, 64, 8), (256, 22, 21, 64, 8), (1344, 257, 7, 64, 8), (2080, 100, 16, 32, 8), (2080, 100, 17, 32, 8),
     (32, 3, 12, 32, 4), (160, 41, 12, 32, 4), (2112, 70, 12, 64, 4), (4096, 300, 16, 64, 4), (1344, 257, 21, 64, 4),
     (1088, 90, 16, 64, 4), (1088, 90, 17, 64, 4),
+    (64, 1, 12, 32, 2), (96, 5, 3, 32, 2), (2112, 70, 16, 64, 2), (160, 41, 17, 32, 2), (1344, 257, 21, 64, 2),
+    (4128, 300, 16, 32, 2),
 ])
 def test_ragged_sizes_against_naive(x, y, steps, word_bits, dw):
     c = ol.bernoulli_cells(x, y, x * 7 + y, 0.4)
@@ -65,7 +67,7 @@ def test_tile_limits():
     with pytest.raises(golhip.GolError):
         golhip.Grid(2048, 2048, kernel=golhip.KERNEL_ACTIVE, temporal_steps=22)
     with pytest.raises(golhip.GolError):
-        golhip.Grid(2048, 2048, kernel=golhip.KERNEL_ACTIVE, lane_dwords=2)
+        golhip.Grid(2048, 2048, kernel=golhip.KERNEL_ACTIVE, lane_dwords=1)
 
 
 def test_skips_tiles_on_a_stable_grid():

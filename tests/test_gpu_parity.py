@@ -261,6 +261,7 @@ def test_persistent_used_for_16384_and_split_calls():
     (1000, dict(kernel=golhip.KERNEL_TEMPORAL, flags=golhip.FLAG_NO_PERSISTENT)),
     (20000, dict(kernel=golhip.KERNEL_ACTIVE)),
     (20000, dict(kernel=golhip.KERNEL_ACTIVE, lane_dwords=8)),
+    (20000, dict(kernel=golhip.KERNEL_ACTIVE, lane_dwords=2)),
 ])
 def test_spacefiller_fixture(gens, kw):
     f = [r for r in FIX["bitpar_runs"] if r["name"] == "spacefiller-16384" and r["generations"] == gens][0]
