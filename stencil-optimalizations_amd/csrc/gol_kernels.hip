@@ -494,7 +494,15 @@ __global__ __launch_bounds__(64 * kFineWaves) void fine_tile_kernel(const uint32
     if (tr.count_clear && blockIdx.x == 0 && threadIdx.x == 0) *tr.count_clear = 0;
     const int n = list ? *count : dense_tiles;
     const int stride = static_cast<int>(gridDim.x) * kFineWaves;
-    int slot = blockIdx.x * kFineWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // Consecutive list entries are neighbouring tiles (a wavefront appends its
+    // 3 x 3 neighbourhood at once), which share halo rows. Workgroup b runs on
+    // XCD b % 8, so runs of 32 consecutive slots go to 8 workgroups of one XCD
+    // (one L2), the runs dealt to the XCDs in turn, so that a short list still
+    // spreads over all of them. A bijection onto the workgroups when their
+    // number is a multiple of 64; else workgroup b takes slots 4b .. 4b + 3.
+    const int b = static_cast<int>(blockIdx.x);
+    const int group = (gridDim.x & 63) == 0 ? ((b >> 6) * 8 + (b & 7)) * 8 + ((b >> 3) & 7) : b;
+    int slot = group * kFineWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (slot >= n) return;
     auto tile_at = [&](int s) { return list ? __builtin_amdgcn_readfirstlane(list[s]) : s; };
     int tile = tile_at(slot);
