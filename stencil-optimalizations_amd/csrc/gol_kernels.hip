@@ -496,12 +496,13 @@ __global__ __launch_bounds__(64 * kFineWaves) void fine_tile_kernel(const uint32
     const int stride = static_cast<int>(gridDim.x) * kFineWaves;
     // Consecutive list entries are neighbouring tiles (a wavefront appends its
     // 3 x 3 neighbourhood at once), which share halo rows. Workgroup b runs on
-    // XCD b % 8, so runs of 32 consecutive slots go to 8 workgroups of one XCD
-    // (one L2), the runs dealt to the XCDs in turn, so that a short list still
-    // spreads over all of them. A bijection onto the workgroups when their
-    // number is a multiple of 64; else workgroup b takes slots 4b .. 4b + 3.
+    // XCD b % 8, so runs of 128 consecutive slots go to 32 workgroups of one
+    // XCD (one L2), the runs dealt to the XCDs in turn, so that a short list
+    // still spreads over all of them. A bijection onto the workgroups when
+    // their number is a multiple of 256; else workgroup b takes slots
+    // 4b .. 4b + 3.
     const int b = static_cast<int>(blockIdx.x);
-    const int group = (gridDim.x & 63) == 0 ? ((b >> 6) * 8 + (b & 7)) * 8 + ((b >> 3) & 7) : b;
+    const int group = (gridDim.x & 255) == 0 ? ((b >> 8) * 8 + (b & 7)) * 32 + ((b >> 3) & 31) : b;
     int slot = group * kFineWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (slot >= n) return;
     auto tile_at = [&](int s) { return list ? __builtin_amdgcn_readfirstlane(list[s]) : s; };
