@@ -481,10 +481,11 @@ __device__ __forceinline__ void fine_tile(const FineRow<DW>& r, uint32_t* __rest
 
 // Wavefronts take slots i, i + stride, ... of tile list[0 .. *count), or of
 // tiles 0 .. dense_tiles - 1 when list is null, and read one tile ahead: the
-// loads of the next tile and the list entry after it are in flight while the
-// current tile advances (a block lists more tiles than the launch has
-// wavefronts when much of the grid is active). The launch resets the counter
-// of the list after next.
+// loads of the next tile are in flight while the current tile advances (a
+// block lists more tiles than the launch has wavefronts when much of the grid
+// is active). The list entry after it is read before the tile advances, a
+// scalar load the wavefront waits for. The launch resets the counter of the
+// list after next.
 template <int DW, bool RING, bool VEC>
 __global__ __launch_bounds__(64 * kFineWaves) void fine_tile_kernel(const uint32_t* __restrict__ src,
                                                                     uint32_t* __restrict__ dst, FineGeo g, int gens,
