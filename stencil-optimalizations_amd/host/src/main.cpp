@@ -65,7 +65,12 @@ std::string first_differences(const infrastructure::Grid<2, cell_t>& a, const in
 
 template <typename cell_t>
 int run_experiment(const ExperimentParams& params) {
-    infrastructure::AlgorithmRepository<2, cell_t> repo;
+    // a 2-D and a 3-D repository, as the reference's experiment manager holds;
+    // its command line has no Z dimension, and the 3-D one stays empty
+    infrastructure::AlgorithmReposCollection<infrastructure::AlgRepoParams<2, cell_t>,
+                                             infrastructure::AlgRepoParams<3, cell_t>>
+        repos;
+    auto& repo = *repos.template get_repository<2, cell_t>();
     algorithms::register_hip_algorithms(repo);
     auto* alg = repo.fetch_algorithm(params.algorithm_name);
     if (!alg) {

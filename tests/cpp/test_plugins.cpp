@@ -60,7 +60,27 @@ static infrastructure::ExperimentParams params_for(std::size_t x, std::size_t y,
     return p;
 }
 
+static void repository_collection_tests() {
+    // a 2-D and a 3-D repository for one cell type, the 3-D one empty
+    infrastructure::AlgorithmReposCollection<infrastructure::AlgRepoParams<2, char>,
+                                             infrastructure::AlgRepoParams<3, char>>
+        repos;
+    auto* r2 = repos.get_repository<2, char>();
+    auto* r3 = repos.get_repository<3, char>();
+    CHECK(r2 != nullptr && r3 != nullptr);
+    CHECK((repos.get_repository<2, std::int32_t>() == nullptr));
+    algorithms::register_hip_algorithms(*r2);
+    CHECK(r2->has_algorithm("gol-hip-temporal-64") && r3->names().empty());
+    int visited = 0, holding = 0;
+    repos.for_each([&](auto& repo) {
+        ++visited;
+        holding += repo.has_algorithm("gol-hip-temporal-64") ? 1 : 0;
+    });
+    CHECK(visited == 2 && holding == 1);
+}
+
 static void cpu_tests() {
+    repository_collection_tests();
     // x-fastest layout: grid[x][y] is element y * X + x
     Grid g(5, 3);
     g[4][2] = 1;
