@@ -70,6 +70,7 @@ struct ServerJob {
     size_t dwords = 0;
     uint32_t* pinned_in = nullptr;
     uint32_t* pinned_out = nullptr;
+    size_t pinned_off = 0, pinned_block = 0;  // block of the server's pinned region (0 bytes: own allocations)
     hipEvent_t done_ev = nullptr;
     JobState state = kQueued;
     int error = GOL_OK;
@@ -133,6 +134,12 @@ struct gol_server {
     uint8_t* arena = nullptr;
     size_t arena_bytes = 0;
     ArenaAllocator alloc;
+    // pinned host region for the jobs' input and result buffers (taken under
+    // mu), so that a submit does not pin pages; a job that finds no room in
+    // it pins its own
+    uint8_t* pinned = nullptr;
+    size_t pinned_bytes = 0;
+    ArenaAllocator pinned_alloc;
     BatchGrid* d_desc = nullptr;
     BatchGrid* h_desc = nullptr;  // pinned staging, one entry per slot
     std::vector<int> free_slots;
@@ -168,9 +175,15 @@ struct gol_server {
 
 namespace {
 
-void free_job(ServerJob* j) {
-    if (j->pinned_in) (void)hipHostFree(j->pinned_in);
-    if (j->pinned_out) (void)hipHostFree(j->pinned_out);
+// Frees a job's host buffers and event; buffers from the server's pinned
+// region go back to it, so the caller holds s->mu (or is the only thread left).
+void free_job(gol_server* s, ServerJob* j) {
+    if (j->pinned_block) {
+        s->pinned_alloc.release(j->pinned_off, j->pinned_block);
+    } else {
+        if (j->pinned_in) (void)hipHostFree(j->pinned_in);
+        if (j->pinned_out) (void)hipHostFree(j->pinned_out);
+    }
     if (j->done_ev) (void)hipEventDestroy(j->done_ev);
     delete j;
 }
@@ -451,6 +464,7 @@ void destroy_resources(gol_server* s) {
     if (s->d_desc) (void)hipFree(s->d_desc);
     if (s->h_desc) (void)hipHostFree(s->h_desc);
     if (s->arena) (void)hipFree(s->arena);
+    if (s->pinned) (void)hipHostFree(s->pinned);
 }
 
 }  // namespace
@@ -490,6 +504,15 @@ int gol_server_create(const gol_server_config* config, gol_server** out) {
         delete s;
         return rc;
     }
+    // a thirty-second of the HBM budget, at most 1 GiB; without it every job
+    // pins its own buffers
+    s->pinned_bytes = std::min<size_t>(size_t(1) << 30, (s->arena_bytes / 32) & ~size_t(255));
+    if (s->pinned_bytes &&
+        hipHostMalloc(reinterpret_cast<void**>(&s->pinned), s->pinned_bytes, hipHostMallocDefault) != hipSuccess) {
+        s->pinned = nullptr;
+        s->pinned_bytes = 0;
+    }
+    s->pinned_alloc.reset(s->pinned_bytes);
     for (auto& L : s->lanes) {
         if ((e = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking)) != hipSuccess ||
             (e = hipEventCreateWithFlags(&L.round_ev[0], hipEventDisableTiming)) != hipSuccess ||
@@ -531,20 +554,33 @@ int gol_server_submit(gol_server* s, uint64_t x_cells, uint64_t y_cells, uint32_
         delete j;
         return rc;
     }
-    hipError_t e;
-    if ((e = hipHostMalloc(reinterpret_cast<void**>(&j->pinned_in), j->dwords * 4, hipHostMallocDefault)) !=
-            hipSuccess ||
-        (e = hipHostMalloc(reinterpret_cast<void**>(&j->pinned_out), j->dwords * 4, hipHostMallocDefault)) !=
-            hipSuccess ||
+    hipError_t e = hipSuccess;
+    const size_t half = (j->dwords * 4 + 255) & ~size_t(255);
+    if (s->pinned) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        size_t off = 0;
+        if (s->pinned_alloc.alloc(2 * half, &off)) {
+            j->pinned_off = off;
+            j->pinned_block = 2 * half;
+            j->pinned_in = reinterpret_cast<uint32_t*>(s->pinned + off);
+            j->pinned_out = reinterpret_cast<uint32_t*>(s->pinned + off + half);
+        }
+    }
+    if ((!j->pinned_block &&
+         ((e = hipHostMalloc(reinterpret_cast<void**>(&j->pinned_in), j->dwords * 4, hipHostMallocDefault)) !=
+              hipSuccess ||
+          (e = hipHostMalloc(reinterpret_cast<void**>(&j->pinned_out), j->dwords * 4, hipHostMallocDefault)) !=
+              hipSuccess)) ||
         (e = hipEventCreateWithFlags(&j->done_ev, hipEventDisableTiming)) != hipSuccess) {
         rc = fail_hip(e, "gol_server_submit");
-        free_job(j);
+        std::lock_guard<std::mutex> lk(s->mu);
+        free_job(s, j);
         return rc;
     }
     std::memcpy(j->pinned_in, words, j->dwords * 4);
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) {
-        free_job(j);
+        free_job(s, j);
         return fail(GOL_ERR_RUNTIME, "server is stopped");
     }
     j->id = s->next_id++;
@@ -594,7 +630,8 @@ int gol_server_wait(gol_server* s, uint64_t job_id, void* words, size_t n_words,
         stats->rounds = j->rounds;
         stats->reserved = 0;
     }
-    free_job(j);
+    lk.lock();
+    free_job(s, j);
     return rc;
 }
 
@@ -618,7 +655,7 @@ void gol_server_destroy(gol_server* s) {
     if (s->worker.joinable()) s->worker.join();
     (void)hipSetDevice(s->device);
     destroy_resources(s);
-    for (auto& kv : s->jobs) free_job(kv.second);
+    for (auto& kv : s->jobs) free_job(s, kv.second);
     delete s;
 }
 
