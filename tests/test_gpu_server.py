@@ -74,6 +74,22 @@ def test_budget_admission_queues_and_reuses_memory():
         assert s.completed == len(specs)
 
 
+def test_pinned_region_fills_then_reuses():
+    # a 64 MiB budget gives a 2 MiB pinned host region: 32 jobs of 512^2 (2 x
+    # 32 KiB of host buffers each) fill it, the rest pin their own buffers;
+    # after their results are taken, later jobs take the freed blocks again
+    specs = [(512, 512, 40 + i, 0.35, 70 + i) for i in range(48)]
+    inputs = [golhip.fill_bernoulli(x, y, sd, p) for x, y, g, p, sd in specs]
+    with golhip.Server(hbm_budget_bytes=64 << 20) as srv:
+        note("pinned region")
+        for rnd in range(2):
+            ids = [srv.submit(w, x, y, g) for w, (x, y, g, p, sd) in zip(inputs, specs)]
+            for i, (x, y, g, p, sd) in enumerate(specs):
+                out, _ = srv.wait(ids[i])
+                assert np.array_equal(out, alone(inputs[i], x, y, g)), (rnd, i)
+        assert srv.stats().completed == 2 * len(specs)
+
+
 def test_oversize_job_fails_alone():
     with golhip.Server(hbm_budget_bytes=256 * 1024) as srv:
         big = golhip.fill_random(2048, 2048, 1)
