@@ -33,6 +33,11 @@ GENERATIONS = 10000
 SEED = 42
 BYTES_PER_CELL_UPDATE = 0.25  # 1 bit read + 1 bit written of packed state per cell per generation
 HBM_PEAK_GBS = 8000.0
+# Integer VALU peak: 256 CUs x 64 lanes x 2.4 GHz of 32-bit lane-ops (one VALU instruction per lane per clock).
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+# The next-state network: 12 VALU ops per 32-cell dword per generation (4 for the row sums, 6 bitop3
+# for the count and the rule, 2 funnel shifts), no halo rows: the algorithmic work per cell-update.
+USEFUL_LANE_OPS_PER_CELL_UPDATE = 12 / 32
 
 
 def log(*a):
@@ -57,6 +62,39 @@ def cpu_baseline(sample_generations: int) -> dict:
         "sample": f"2048x2048 random seed 42, {sample_generations} of the 1000 generations of the "
                   f"reference CPU workload, naive oracle loop (char cells, dead boundary), "
                   f"{dt:.1f} s on 1 of {os.cpu_count()} host threads",
+    }
+
+
+def cpu_baseline_bitparallel(min_seconds: float = 3.0) -> dict:
+    """The oracle's OpenMP bit-parallel engine (uint64 rows) on every host thread
+    the process may use, on the reference's CPU workload (2048^2 seed-42 random,
+    1000 generations), repeated until min_seconds have passed. A second,
+    labelled baseline: faster than the reference's single-threaded loop."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1)
+    words = ol.pack(ol.random_cells(2048, 2048, 42), 64)
+    ol.bitpar_run(words, 2048, 2048, 10, threads)  # warm-up: thread pool start
+    runs = 0
+    t0 = time.perf_counter()
+    while True:
+        out = ol.bitpar_run(words, 2048, 2048, 1000, threads)
+        runs += 1
+        dt = time.perf_counter() - t0
+        if dt >= min_seconds:
+            break
+    want = next((r for r in json.load(open(os.path.join(ROOT, "tests", "golden", "fixtures.json")))["naive_runs"]
+                 if (r["x"], r["generations"], r["generator"]) == (2048, 1000, "random")), None)
+    exact = ol.sha256(out) == want["sha256"] if want else None
+    return {
+        "value": 2048 * 2048 * 1000 * runs / dt,
+        "unit": "cell-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"2048x2048 random seed 42 x 1000 generations (the reference's CPU workload), {runs} runs in "
+                  f"{dt:.1f} s, the oracle's OpenMP bit-parallel uint64-row engine on {threads} threads "
+                  f"(nproc {os.cpu_count()}); final grid bit-exact vs the naive-loop fixture: {exact}",
     }
 
 
@@ -244,6 +282,18 @@ def leg_batched(n_grids: int, plain_sample: int) -> dict:
     }
 
 
+def serve_alone(specs, inputs):
+    """Each job alone on an idle server, one after another (submit, wait, next):
+    its latency with the whole GPU to itself, the denominator of its slowdown."""
+    lat = []
+    with golhip.Server(device=0, hbm_budget_bytes=32 << 30) as srv:
+        srv.wait(srv.submit(golhip.fill_random(256, 256, 1), 256, 256, 10))
+        for s, w in zip(specs, inputs):
+            _, st = srv.wait(srv.submit(w, s.side, s.side, s.generations))
+            lat.append(st.latency_ms)
+    return lat
+
+
 def serve(specs, inputs, paced: bool):
     """Submit the jobs (at their arrival times when paced, else all at once) to a
     fresh server and wait for every result; returns (results, latencies, queue
@@ -338,6 +388,11 @@ def leg_served(n_jobs: int, load: float) -> dict:
         plain_updates += s.cell_updates
     plain_rate = plain_updates / plain_wall
     lat_sorted = sorted(lat)
+    # per-job slowdown: latency in the paced mix / latency alone on an idle server
+    alone = serve_alone(specs, inputs)
+    slowdown = np.array(lat) / np.array(alone)
+    work = np.array([s.cell_updates for s in specs], dtype=np.float64)
+    small = work <= np.median(work)
     return {
         "workload": f"{n_jobs} jobs, Poisson arrivals at {rate:.1f}/s (load {load:.2f} of the server's "
                     f"capacity {capacity:.2e} cell-updates/s, measured with every job submitted at once), "
@@ -350,6 +405,14 @@ def leg_served(n_jobs: int, load: float) -> dict:
         "latency_p99_ms": float(np.percentile(lat_sorted, 99)),
         "latency_max_ms": lat_sorted[-1],
         "queue_p50_ms": float(np.percentile(queue_ms, 50)),
+        "slowdown_p50": float(np.percentile(slowdown, 50)),
+        "slowdown_p99": float(np.percentile(slowdown, 99)),
+        "slowdown_max": float(slowdown.max()),
+        "slowdown_small_jobs_p50": float(np.percentile(slowdown[small], 50)),
+        "slowdown_small_jobs_p99": float(np.percentile(slowdown[small], 99)),
+        "slowdown_note": "latency in the paced mix / latency of the same job alone on an idle server "
+                         "(submit -> result in host memory); small jobs: the half with the least work",
+        "alone_latency_p50_ms": float(np.percentile(alone, 50)),
         "rounds": stats.rounds,
         "resident_bytes_peak": stats.resident_bytes_peak,
         "sample_bit_exact_alone_vs_served": len(sample),
@@ -449,23 +512,32 @@ def main() -> int:
     cells = X * Y
     total_updates = cells * GENERATIONS * args.steps
     value = total_updates / wall
-    # roofline of the dominant kernel: algorithmic bytes per launch / average launch duration
+    # Roofline of the dominant kernel. The kernel is integer bit logic bound by VALU issue, not by HBM
+    # (it moves 3 % of its algorithmic bytes). Per launch, from the committed rocprofv3 profile of this
+    # launch form (used only when its shape, depth and generations per launch match this run):
+    # SQ_INSTS_VALU (wavefront instructions, x 64 lanes) and FETCH_SIZE / WRITE_SIZE (calibrated
+    # against a copy kernel). The launch duration is this run's own, from HIP events on the stream.
     avg_launch_s = (gpu_ms / 1e3) / launches
-    bytes_per_launch = total_updates * BYTES_PER_CELL_UPDATE / launches
-    achieved_gbs = bytes_per_launch / avg_launch_s / 1e9
-    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes
-    # (calibrated against a copy kernel), valid for the profiled shape and depth only
-    traffic = None
+    updates_per_launch = total_updates / launches
+    profiled = {}
     prof = os.path.join(ROOT, "profiles", "slab_kernel_16384x16384_random.md" if info.persistent
                         else "tile_kernel_16384x16384_random.md")
     if os.path.exists(prof) and args.kernel == "temporal":
         import re
         text = open(prof).read()
-        m = re.search(r"hbm_bytes_per_launch:\s*(\d+)", text)
-        s = re.search(r"profiled_shape:\s*(\d+)\s+(\d+)\s+(\d+)\s+(\d+)", text)
-        if m and s and tuple(int(v) for v in s.groups()) == (info.lane_dwords, info.band_rows, info.waves,
-                                                                 info.temporal_steps):
-            traffic = int(m.group(1))
+        shape = re.search(r"profiled_shape:\s*(\d+)\s+(\d+)\s+(\d+)\s+(\d+)\s+(\d+)", text)
+        if shape and tuple(int(v) for v in shape.groups()) == (info.lane_dwords, info.band_rows, info.waves,
+                                                                   info.temporal_steps,
+                                                                   GENERATIONS * args.steps // launches):
+            for key in ("hbm_bytes_per_launch", "valu_insts_per_launch"):
+                m = re.search(key + r":\s*([0-9.e+]+)", text)
+                if m:
+                    profiled[key] = float(m.group(1))
+    valu = profiled.get("valu_insts_per_launch")
+    traffic = profiled.get("hbm_bytes_per_launch")
+    issued_tops = valu * 64 / avg_launch_s / 1e12 if valu else None
+    useful_tops = updates_per_launch * USEFUL_LANE_OPS_PER_CELL_UPDATE / avg_launch_s / 1e12
+    normalised_gbs = updates_per_launch * BYTES_PER_CELL_UPDATE / avg_launch_s / 1e9
 
     result = {
         "metric": METRIC,
@@ -479,6 +551,8 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
+        "dtype_note": "bit logic on 32-bit lane words: each rows-64 word is processed as its two dwords, "
+                      "same bytes, no width lost",
         "data": "synthetic",
         "config": {
             "workload": f"{X}x{Y} random (std::mt19937 seed {SEED}, p=0.5), {GENERATIONS} generations "
@@ -491,13 +565,25 @@ def main() -> int:
             "bit_exact_vs_oracle": exact,
         },
         "roofline": {
-            "bound": "hbm",
-            "achieved": achieved_gbs,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved_gbs / HBM_PEAK_GBS,
+            "bound": "valu",
+            "achieved": issued_tops,
+            "peak": VALU_PEAK_TOPS,
+            "unit": "TOP/s (32-bit integer lane-ops)",
+            "frac": issued_tops / VALU_PEAK_TOPS if issued_tops else None,
+            "valu_insts_per_launch": valu,
+            "useful_achieved": useful_tops,
+            "useful_frac": useful_tops / VALU_PEAK_TOPS,
+            "useful_note": f"{USEFUL_LANE_OPS_PER_CELL_UPDATE} lane-ops per cell-update (the 12-op network per "
+                           f"32-cell dword, no halo rows) x cell-updates per launch / launch time",
             "traffic": traffic,
+            "hbm_achieved_gbs": traffic / avg_launch_s / 1e9 if traffic else None,
+            "hbm_frac": traffic / avg_launch_s / 1e9 / HBM_PEAK_GBS if traffic else None,
+            "counters_from": os.path.relpath(prof, ROOT) if profiled else None,
         },
+        "hbm_equivalent_gbs": normalised_gbs,
+        "hbm_equivalent_vs_peak": normalised_gbs / HBM_PEAK_GBS,
+        "hbm_equivalent_note": "cell-updates/s x 0.25 B (1 bit read + 1 bit written per cell-update) against "
+                               "8 TB/s: exceeds 1 under temporal blocking, not a roofline fraction",
         "gpu_ms_per_step": gpu_ms / args.steps,
         "pcie_inclusive_cell_updates_per_s": X * Y * GENERATIONS / pcie_s,
         "avg_launch_us": avg_launch_s * 1e6,
@@ -526,6 +612,8 @@ def main() -> int:
     if not args.no_cpu_baseline:
         log("timing the CPU baseline sample ...")
         result["cpu_baseline"] = cpu_baseline(args.cpu_sample_generations)
+        log("timing the bit-parallel CPU baseline ...")
+        result["cpu_baseline_bitparallel"] = cpu_baseline_bitparallel()
     print(json.dumps(result), flush=True)
     return 0
 

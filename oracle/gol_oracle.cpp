@@ -38,7 +38,9 @@
 //                           only where the naive loop is too slow (16384^2 x 10^4).
 //
 // Each function is pinned by tests/test_oracle.py: the AN5D alive-count KATs
-// (AN5D/GOL_AN5D.c:92-97), the C++ standard's mt19937 10000th-output value,
+// (AN5D/GOL_AN5D.c:92-97), the C++ standard's mt19937 and mt19937_64
+// 10000th-output values, the loaders' cells against pure-Python restatements
+// of both engines and of the two cell mappings,
 // exhaustive 3x3 neighbourhoods against the B3/S23 rule table, still lifes,
 // oscillators, glider translation, and the D4 symmetries of the grid.
 
@@ -68,6 +70,14 @@ void oracle_load_random(uint8_t* cells, uint64_t x_size, uint64_t y_size, uint64
 uint32_t oracle_mt19937_nth(uint64_t seed, uint64_t n) {
     std::mt19937 rng(static_cast<std::mt19937::result_type>(seed));
     uint32_t v = 0;
+    for (uint64_t i = 0; i < n; ++i) v = rng();
+    return v;
+}
+
+// The n-th output of std::mt19937_64(seed), the engine of oracle_load_bernoulli.
+uint64_t oracle_mt19937_64_nth(uint64_t seed, uint64_t n) {
+    std::mt19937_64 rng(seed);
+    uint64_t v = 0;
     for (uint64_t i = 0; i < n; ++i) v = rng();
     return v;
 }
@@ -334,12 +344,6 @@ void oracle_bitpar_run(uint64_t* words, uint64_t x_size, uint64_t y_size, uint64
         dst = t;
     }
     if (src != words) std::memcpy(words, src, n * sizeof(uint64_t));
-}
-
-uint64_t oracle_popcount_words(const uint64_t* words, uint64_t n) {
-    uint64_t c = 0;
-    for (uint64_t i = 0; i < n; ++i) c += static_cast<uint64_t>(__builtin_popcountll(words[i]));
-    return c;
 }
 
 }  // extern "C"

@@ -29,6 +29,8 @@ def lib() -> ctypes.CDLL:
         L.oracle_load_random.argtypes = [vp, u64, u64, u64]
         L.oracle_mt19937_nth.argtypes = [u64, u64]
         L.oracle_mt19937_nth.restype = ctypes.c_uint32
+        L.oracle_mt19937_64_nth.argtypes = [u64, u64]
+        L.oracle_mt19937_64_nth.restype = ctypes.c_uint64
         L.oracle_load_bernoulli.argtypes = [vp, u64, u64, ctypes.c_double]
         L.oracle_load_glibc_rand.argtypes = [vp, u64, ctypes.c_uint]
         L.oracle_naive_run.argtypes = [vp, u64, u64, u64]
@@ -38,8 +40,6 @@ def lib() -> ctypes.CDLL:
         L.oracle_pack_layout.argtypes = [vp, u64, u64, ctypes.c_uint32, ctypes.c_uint32, vp]
         L.oracle_unpack_layout.argtypes = [vp, u64, u64, ctypes.c_uint32, ctypes.c_uint32, vp]
         L.oracle_bitpar_run.argtypes = [vp, u64, u64, u64, ctypes.c_int]
-        L.oracle_popcount_words.argtypes = [vp, u64]
-        L.oracle_popcount_words.restype = ctypes.c_uint64
         _lib = L
     return _lib
 

@@ -10,7 +10,12 @@ Each check is chosen so that a plausible mistake in the restatement fails it:
   * the eight symmetries of the square (a transposed operand or a wrong
     index fails these);
   * the bit-parallel engine against the naive loop on ragged sizes;
-  * the loaders: mt19937's standard 10000th value and the seed-42 fixture.
+  * the loaders: pure-Python mt19937 / mt19937_64 written from the C++
+    standard's parameters (pinned by the standard's 10000th values) against
+    the oracle's loader cells, the Bernoulli threshold in exact rationals,
+    and the seed-42 fixture;
+  * the batched / served workload fixtures: their smallest grids re-derived
+    with the naive loop.
 """
 
 import json
@@ -31,6 +36,105 @@ B3S23 = np.array([[0, 0, 0, 1, 0, 0, 0, 0, 0],
 def test_mt19937_standard_value():
     # [rand.predef]: the 10000th invocation of a default-constructed mt19937 is 4123659995
     assert ol.lib().oracle_mt19937_nth(5489, 10000) == FIX["mt19937_10000"] == 4123659995
+
+
+class MersenneTwister:
+    """The Mersenne twister family of [rand.eng.mers], written from the C++
+    standard's parameters (not from any library's code): mt19937 and
+    mt19937_64 of [rand.predef]. Independent of the oracle, which calls
+    std::mt19937 / std::mt19937_64."""
+
+    def __init__(self, w, n, m, r, a, u, d, s, b, t, c, l, f, seed):
+        self.w, self.n, self.m, self.r, self.a = w, n, m, r, a
+        self.u, self.d, self.s, self.b, self.t, self.c, self.l = u, d, s, b, t, c, l
+        self.mask = (1 << w) - 1
+        x = [seed & self.mask]
+        for i in range(1, n):
+            x.append((f * (x[-1] ^ (x[-1] >> (w - 2))) + i) & self.mask)
+        self.x, self.i = x, n
+
+    def __call__(self):
+        if self.i == self.n:
+            upper = (self.mask << self.r) & self.mask
+            lower = (1 << self.r) - 1
+            x = self.x
+            for k in range(self.n):
+                y = (x[k] & upper) | (x[(k + 1) % self.n] & lower)
+                x[k] = x[(k + self.m) % self.n] ^ (y >> 1) ^ (self.a if y & 1 else 0)
+            self.i = 0
+        z = self.x[self.i]
+        self.i += 1
+        z ^= (z >> self.u) & self.d
+        z ^= (z << self.s) & self.b
+        z ^= (z << self.t) & self.c
+        z ^= z >> self.l
+        return z & self.mask
+
+
+def mt19937(seed=5489):
+    return MersenneTwister(32, 624, 397, 31, 0x9908B0DF, 11, 0xFFFFFFFF, 7, 0x9D2C5680, 15, 0xEFC60000, 18,
+                           1812433253, seed)
+
+
+def mt19937_64(seed=5489):
+    return MersenneTwister(64, 312, 156, 31, 0xB5026F5AA96619E9, 29, 0x5555555555555555, 17, 0x71D67FFFEDA60000,
+                           37, 0xFFF7EEE000000000, 43, 6364136223846793005, seed)
+
+
+def nth(engine, n):
+    for _ in range(n - 1):
+        engine()
+    return engine()
+
+
+def test_python_engines_match_the_standard():
+    # [rand.predef]: the 10000th outputs of default-constructed mt19937 / mt19937_64
+    assert nth(mt19937(), 10000) == 4123659995
+    assert nth(mt19937_64(), 10000) == 9981545732273789042
+
+
+def test_mt19937_64_standard_value():
+    # the engine of oracle_load_bernoulli
+    assert ol.lib().oracle_mt19937_64_nth(5489, 10000) == 9981545732273789042
+    assert ol.lib().oracle_mt19937_64_nth(99, 1000) == nth(mt19937_64(99), 1000)
+
+
+@pytest.mark.parametrize("seed,p", [(99, 0.35), (7, 0.5), (123456789, 0.1), (2 ** 63 + 5, 0.9)])
+def test_bernoulli_loader_against_python_restatement(seed, p):
+    # cell i = u_i < p, u_i = (x_i >> 11) / 2^53 for the i-th mt19937_64 output
+    # x_i. Computed in exact rationals: (x >> 11) < p * 2^53, so a 32-bit
+    # engine, a shift of 12, or <= in place of < fails on some cell.
+    from fractions import Fraction
+
+    n = 4096
+    got = ol.bernoulli_cells(64, n // 64, seed, p).reshape(-1)
+    eng = mt19937_64(seed)
+    limit = Fraction(p) * 2 ** 53
+    want = np.array([1 if (eng() >> 11) < limit else 0 for _ in range(n)], np.uint8)
+    assert np.array_equal(got, want)
+
+
+def test_bernoulli_loader_threshold_is_strict():
+    # p equal to a cell's own u: the cell is dead (u < p fails), and alive for
+    # the next double above p
+    seed = 31
+    x = mt19937_64(seed)()
+    u = (x >> 11) / 2.0 ** 53  # exact: 53-bit integer over a power of two
+    assert ol.bernoulli_cells(64, 1, seed, u)[0, 0] == 0
+    assert ol.bernoulli_cells(64, 1, seed, float(np.nextafter(u, 2.0)))[0, 0] == 1
+
+
+def test_random_loader_against_python_restatement():
+    # RandomOnesZerosDataLoader (data_loader.hpp:57-75) draws
+    # uniform_int_distribution<int>(0, 1) from mt19937(seed). libstdc++ maps a
+    # 32-bit draw x onto {0, 1} as (x * 2) >> 32 (Lemire's multiply-shift,
+    # with no rejection possible for a range of 2): the top bit of x. The seed-42
+    # fixture holds the oracle's first 256 cells; here the whole first 64 K.
+    n = 65536
+    got = ol.random_cells(256, n // 256, 42).reshape(-1)
+    eng = mt19937(42)
+    want = np.array([(eng() * 2) >> 32 for _ in range(n)], np.uint8)
+    assert np.array_equal(got, want)
 
 
 def test_loader_fixture():
@@ -232,3 +336,24 @@ def test_naive_fixtures_small():
         out = ol.naive_run(g, f["generations"])
         assert ol.sha256(ol.pack(out, f["word_bits"])) == f["sha256"], f
         assert int(out.sum()) == f["popcount"]
+
+
+def test_workload_fixtures_against_the_naive_loop():
+    # The batched / served workload fixtures come from the bit-parallel engine;
+    # re-derive the smallest grids of each with the naive loop, from the
+    # oracle's loader, and check every grid's parameters against the workload
+    # generators.
+    import workloads
+
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "workload_fixtures.json")))
+    for name, specs in (("batched_uneven", workloads.batched_uneven(fx["batched_uneven"]["n"])),
+                        ("served_stream", workloads.served_stream(fx["served_stream"]["n"], 1.0))):
+        grids = fx[name]["grids"]
+        assert [(s.side, s.generations) for s in specs] == [tuple(g[:2]) for g in grids[:len(specs)]]
+        for i in sorted(range(len(specs)), key=lambda i: specs[i].cell_updates)[:4]:
+            s = specs[i]
+            cells = ol.bernoulli_cells(s.side, s.side, s.seed, s.density)
+            assert ol.sha256(ol.pack(cells, 64))[:16] == grids[i][2]
+            out = ol.naive_run(cells, s.generations)
+            assert ol.sha256(ol.pack(out, 64))[:16] == grids[i][3], (name, i)
+            assert int(out.sum()) == grids[i][4]
