@@ -161,6 +161,29 @@ int gol_get_info(gol_handle* h, gol_info* info);
 /* Release the device state (reference: cudaFree in finalize_data_structures). */
 void gol_destroy(gol_handle* h);
 
+/* Advance a grid held in caller-owned device memory without allocating
+ * anything: no hipMalloc, hipFree or allocating library routine. No counterpart
+ * in the reference; it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation). It is gol_step's persistent slab launch (GOL_KERNEL_TEMPORAL)
+ * with the launch's scratch, the exchanged halo rows, per-slab block
+ * counters and an error word, in a file-scope device array of 16 MiB + 2 KiB
+ * that each GPU the library runs on holds until the process ends.
+ *   device_in  rows-layout words of the x_cells x y_cells grid (read only);
+ *   device_out receives the grid after `generations` generations; it must not
+ *              overlap device_in;
+ *   stream     hipStream_t to run on (NULL: the null stream), on the current
+ *              device; the call waits for it before returning.
+ * Limits (GOL_ERR_INVALID_ARGUMENT beyond them): rows of at most 16384 cells,
+ * grids the persistent slab launch takes on the device (at least two slabs,
+ * every slab resident at once, at most 256), at most 65536 blocks of
+ * generations per call (524288 at the usual 8 per block: call again with the
+ * output as the input for more). Calls that overlap in time on one GPU share
+ * its scratch and race: serialise them. A stopped launch (a slab's wait ran
+ * out) returns GOL_ERR_RUNTIME. */
+int gol_step_device_noalloc(const void* device_in, void* device_out, uint64_t x_cells, uint64_t y_cells,
+                            uint64_t generations, void* stream);
+
 /* ---- Batches: many independent grids of uneven size, each with its own
  * generation count (no reference counterpart: the reference runs one grid per
  * process, src/infrastructure/experiment_manager.hpp:235-270). Each grid's

@@ -37,4 +37,9 @@ int plan_batch_grid(int64_t x_dw, int64_t y_rows, int round_generations, Planner
                     TileGeometry* geom, int* per_round);
 bool shape_compiled(const TileShape& s);
 
+// Time model of a temporal launch (us per generation) and whether the
+// persistent slab launch takes a shape and depth for a grid (gol_capi.hip).
+double model_us_per_generation(const TileShape& s, const TileGeometry& g, int steps, int occupancy, bool persistent);
+bool slab_fits(const TileShape& s, const TileGeometry& g, int64_t x_dw, int steps);
+
 }  // namespace golhip

@@ -34,7 +34,7 @@ KERNEL_ACTIVE = 3
 # every symbol include/golhip.h declares
 EXPORTED_SYMBOLS = (
     "gol_create", "gol_upload", "gol_upload_device", "gol_step", "gol_step_timed", "gol_download", "gol_sync",
-    "gol_device_state", "gol_get_stream", "gol_get_info", "gol_destroy", "gol_last_error",
+    "gol_device_state", "gol_get_stream", "gol_get_info", "gol_destroy", "gol_step_device_noalloc", "gol_last_error",
     "gol_version", "gol_fill_random", "gol_fill_bernoulli", "gol_fill_patches", "gol_fill_lexicon",
     "gol_batch_create", "gol_batch_add", "gol_batch_run", "gol_batch_reset", "gol_batch_fetch",
     "gol_batch_get_info", "gol_batch_destroy",
@@ -199,6 +199,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.gol_get_info.argtypes = [vp, ctypes.POINTER(GolInfo)]
     lib.gol_destroy.argtypes = [vp]
     lib.gol_destroy.restype = None
+    lib.gol_step_device_noalloc.argtypes = [vp, vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, vp]
+    lib.gol_step_device_noalloc.restype = ctypes.c_int
     lib.gol_last_error.restype = ctypes.c_char_p
     lib.gol_version.restype = ctypes.c_char_p
     u64 = ctypes.c_uint64
@@ -454,6 +456,15 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def step_device_noalloc(device_in: int, device_out: int, x_cells: int, y_cells: int, generations: int,
+                        stream: int = 0) -> None:
+    """gol_step_device_noalloc: advance the rows-layout grid at device address
+    device_in by `generations` into device_out with no device allocation (the
+    persistent slab launch, its scratch in a static device array); waits for
+    the stream."""
+    _check(load_library().gol_step_device_noalloc(device_in, device_out, x_cells, y_cells, generations, stream))
 
 
 def _words(x_cells: int, y_cells: int, word_bits: int) -> np.ndarray:

@@ -137,3 +137,18 @@ def test_python_pack_matches_oracle():
     assert np.array_equal(golhip.pack_rows(c, 64).reshape(-1), ol.pack(c, 64))
     assert np.array_equal(golhip.pack_rows(c, 32).reshape(-1), ol.pack(c, 32))
     assert np.array_equal(golhip.unpack_rows(ol.pack(c, 64), 192, 7), c)
+
+
+@pytest.mark.parametrize("args,message", [
+    ((0, 1 << 20, 16384, 64, 10), "null device buffer"),
+    ((1 << 20, 2 << 20, 100, 64, 10), "multiple of 32"),
+    ((1 << 20, 2 << 20, 32768, 64, 10), "at most 16384 cells"),
+    ((1 << 20, (1 << 20) + 4096, 16384, 64, 10), "overlap"),
+])
+def test_noalloc_errors_before_device(args, message):
+    # argument errors of gol_step_device_noalloc come before any HIP call (the
+    # addresses are never touched)
+    with pytest.raises(golhip.GolError) as e:
+        golhip.step_device_noalloc(*args)
+    assert e.value.code == golhip.GOL_ERR_INVALID_ARGUMENT
+    assert message in str(e.value)
