@@ -317,6 +317,10 @@ typedef struct gol_server_stats {
     uint64_t resident_bytes;      /* arena bytes in use now */
     uint64_t resident_bytes_peak;
     uint32_t resident_jobs, queued_jobs;
+    uint64_t pinned_region_bytes; /* the server's pinned host region for jobs' input and result buffers */
+    uint64_t pinned_region_used;  /* bytes of it held by jobs now */
+    uint64_t pinned_region_peak;
+    uint64_t pinned_own_jobs;     /* jobs that found no room in the region and pinned their own buffers */
 } gol_server_stats;
 
 /* config may be NULL (defaults above). Allocates the budget up front. */

@@ -257,6 +257,7 @@ struct gol_batch {
     bool queue = false;
     unsigned int* d_qstate = nullptr;
     int qstate_words = 0;
+    unsigned long long wait_ticks = 0;  // persistent queue: device wall-clock ticks a wait may last
     bool dirty = false;                // a run changed the buffers since the initial states were restored
     bool result_parity_valid = false;  // a run completed since then: BatchEntry::result_parity holds
     // GOL_BATCH_SELECT_PLAN: the two plans' timed runs and which one was kept
@@ -549,6 +550,7 @@ int enqueue_rounds(gol_batch* b) {
             q.items = b->d_items + L.first;
             q.n_items = L.count;
             q.spin_limit = probe ? 256 : 0;
+            q.wait_ticks = b->wait_ticks;
             q.next = b->d_qstate + k;
             q.done = b->d_qstate + n_launches;
             q.error = b->d_qstate + b->qstate_words - 1;
@@ -663,6 +665,12 @@ int gol_batch_create(const gol_batch_config* config, gol_batch** out) {
     b->queue = (cfg.flags & GOL_BATCH_PERSISTENT) != 0;
     b->device = dev;
     Guard g(dev);
+    // a queue wait gives up after 20 s of device time (the wall clock's rate
+    // in kHz; 100 MHz if the device does not report it)
+    int clock_khz = 0;
+    if (hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess || clock_khz <= 0)
+        clock_khz = 100000;
+    b->wait_ticks = static_cast<unsigned long long>(clock_khz) * 1000ull * 20ull;
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming)) != hipSuccess) {

@@ -671,20 +671,24 @@ __global__ __launch_bounds__(64 * kNarrowMaxWaves) void narrow_item_kernel(const
 // those writes back or invalidates a whole L2, which made the 4096-grid batch
 // 11 % slower (209 ms against 188 ms). Every item is queued after the items it waits for, so the
 // unfinished item with the lowest ticket can always run and the queue drains.
-// Waits are bounded: one that runs out raises *error, and every workgroup
-// leaves at its next step.
+// Waits are bounded by device time: one that runs out raises *error, and
+// every workgroup leaves at its next step. A wait can legitimately last as
+// long as two whole-grid items of up to 2^20 generations (its predecessor may
+// be the next ticket of a workgroup still running another item), so the bound
+// is elapsed time, not a poll count.
 // ---------------------------------------------------------------------------
-constexpr int kQueueSpinLimit = 1 << 20;
 
 // A failed wait raises *error for the host and sets the top bit of the ticket
 // counter, so every later ticket is past the queue's end and each workgroup
 // stops at its next step without reading the error word.
 constexpr unsigned int kTicketStop = 1u << 31;
 
-__device__ bool wait_grid_done(const QueueArgs& q, unsigned int* done, unsigned int target, int spin_limit) {
-    for (int i = 0; i < spin_limit; ++i) {
+__device__ bool wait_grid_done(const QueueArgs& q, unsigned int* done, unsigned int target) {
+    const unsigned long long start = wall_clock64();
+    for (int i = 0;; ++i) {
         if (__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
         if (__hip_atomic_load(q.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return false;
+        if (q.spin_limit > 0 ? i >= q.spin_limit : wall_clock64() - start > q.wait_ticks) break;
         __builtin_amdgcn_s_sleep(2);
     }
     __hip_atomic_fetch_or(q.error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -710,9 +714,7 @@ __device__ __forceinline__ int queue_step(const QueueArgs& q, int prev_grid, uns
                                : __hip_atomic_fetch_add(q.next, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (t >= static_cast<unsigned int>(q.n_items)) return -1;
     const WorkItem& it = q.items[t];
-    const int spins = q.spin_limit > 0 ? q.spin_limit : kQueueSpinLimit;
-    if (it.after > 0 &&
-        !wait_grid_done(q, q.done + it.grid, static_cast<unsigned int>(it.after) + q.wait_ahead, spins))
+    if (it.after > 0 && !wait_grid_done(q, q.done + it.grid, static_cast<unsigned int>(it.after) + q.wait_ahead))
         return -1;
     return static_cast<int>(t);
 }

@@ -151,7 +151,8 @@ struct QueueArgs {
     const BatchGrid* grids;
     const WorkItem* items;
     int n_items;
-    int spin_limit;           // polls before a wait gives up; 0: the built-in limit
+    int spin_limit;           // > 0 (testing): polls before a wait gives up, instead of wait_ticks
+    unsigned long long wait_ticks;  // device wall-clock ticks (wall_clock64) before a wait gives up
     unsigned int* next;       // ticket counter
     unsigned int* done;       // per grid: items completed in this run
     unsigned int* error;

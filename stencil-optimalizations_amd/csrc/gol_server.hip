@@ -175,15 +175,22 @@ struct gol_server {
 
 namespace {
 
-// Frees a job's host buffers and event; buffers from the server's pinned
-// region go back to it, so the caller holds s->mu (or is the only thread left).
-void free_job(gol_server* s, ServerJob* j) {
-    if (j->pinned_block) {
-        s->pinned_alloc.release(j->pinned_off, j->pinned_block);
-    } else {
-        if (j->pinned_in) (void)hipHostFree(j->pinned_in);
-        if (j->pinned_out) (void)hipHostFree(j->pinned_out);
-    }
+// Returns a job's block of the server's pinned region; the caller holds s->mu
+// (or is the only thread left).
+void release_pinned_block(gol_server* s, ServerJob* j) {
+    if (!j->pinned_block) return;
+    s->pinned_alloc.release(j->pinned_off, j->pinned_block);
+    s->stats.pinned_region_used = s->pinned_alloc.used();
+    j->pinned_block = 0;
+    j->pinned_in = j->pinned_out = nullptr;
+}
+
+// Frees what else a job owns: host buffers it pinned itself (when the region
+// had no room) and its event. Called without s->mu held: hipHostFree may wait
+// for the device, which would stall the scheduler and every caller.
+void destroy_job(ServerJob* j) {
+    if (j->pinned_in) (void)hipHostFree(j->pinned_in);
+    if (j->pinned_out) (void)hipHostFree(j->pinned_out);
     if (j->done_ev) (void)hipEventDestroy(j->done_ev);
     delete j;
 }
@@ -513,6 +520,7 @@ int gol_server_create(const gol_server_config* config, gol_server** out) {
         s->pinned_bytes = 0;
     }
     s->pinned_alloc.reset(s->pinned_bytes);
+    s->stats.pinned_region_bytes = s->pinned_bytes;
     for (auto& L : s->lanes) {
         if ((e = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking)) != hipSuccess ||
             (e = hipEventCreateWithFlags(&L.round_ev[0], hipEventDisableTiming)) != hipSuccess ||
@@ -556,14 +564,18 @@ int gol_server_submit(gol_server* s, uint64_t x_cells, uint64_t y_cells, uint32_
     }
     hipError_t e = hipSuccess;
     const size_t half = (j->dwords * 4 + 255) & ~size_t(255);
-    if (s->pinned) {
+    {
         std::lock_guard<std::mutex> lk(s->mu);
         size_t off = 0;
-        if (s->pinned_alloc.alloc(2 * half, &off)) {
+        if (s->pinned && s->pinned_alloc.alloc(2 * half, &off)) {
             j->pinned_off = off;
             j->pinned_block = 2 * half;
             j->pinned_in = reinterpret_cast<uint32_t*>(s->pinned + off);
             j->pinned_out = reinterpret_cast<uint32_t*>(s->pinned + off + half);
+            s->stats.pinned_region_used = s->pinned_alloc.used();
+            s->stats.pinned_region_peak = std::max<uint64_t>(s->stats.pinned_region_peak, s->pinned_alloc.used());
+        } else {
+            s->stats.pinned_own_jobs += 1;
         }
     }
     if ((!j->pinned_block &&
@@ -573,14 +585,19 @@ int gol_server_submit(gol_server* s, uint64_t x_cells, uint64_t y_cells, uint32_
               hipSuccess)) ||
         (e = hipEventCreateWithFlags(&j->done_ev, hipEventDisableTiming)) != hipSuccess) {
         rc = fail_hip(e, "gol_server_submit");
-        std::lock_guard<std::mutex> lk(s->mu);
-        free_job(s, j);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            release_pinned_block(s, j);
+        }
+        destroy_job(j);
         return rc;
     }
     std::memcpy(j->pinned_in, words, j->dwords * 4);
-    std::lock_guard<std::mutex> lk(s->mu);
+    std::unique_lock<std::mutex> lk(s->mu);
     if (s->stop) {
-        free_job(s, j);
+        release_pinned_block(s, j);
+        lk.unlock();
+        destroy_job(j);
         return fail(GOL_ERR_RUNTIME, "server is stopped");
     }
     j->id = s->next_id++;
@@ -631,7 +648,9 @@ int gol_server_wait(gol_server* s, uint64_t job_id, void* words, size_t n_words,
         stats->reserved = 0;
     }
     lk.lock();
-    free_job(s, j);
+    release_pinned_block(s, j);
+    lk.unlock();
+    destroy_job(j);
     return rc;
 }
 
@@ -655,7 +674,10 @@ void gol_server_destroy(gol_server* s) {
     if (s->worker.joinable()) s->worker.join();
     (void)hipSetDevice(s->device);
     destroy_resources(s);
-    for (auto& kv : s->jobs) free_job(s, kv.second);
+    for (auto& kv : s->jobs) {
+        release_pinned_block(s, kv.second);
+        destroy_job(kv.second);
+    }
     delete s;
 }
 

@@ -171,6 +171,8 @@ class GolServerStats(ctypes.Structure):
         ("cell_updates", ctypes.c_uint64),
         ("resident_bytes", ctypes.c_uint64), ("resident_bytes_peak", ctypes.c_uint64),
         ("resident_jobs", ctypes.c_uint32), ("queued_jobs", ctypes.c_uint32),
+        ("pinned_region_bytes", ctypes.c_uint64), ("pinned_region_used", ctypes.c_uint64),
+        ("pinned_region_peak", ctypes.c_uint64), ("pinned_own_jobs", ctypes.c_uint64),
     ]
 
 

@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <iostream>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -148,6 +149,10 @@ class GoLHip : public infrastructure::Algorithm<2, grid_cell_t> {
 
     void run(size_type iterations) override {
         performed_ = 0;
+        if (Kernel == GOL_KERNEL_ACTIVE && this->params.collect_touched_tiles_stats) {
+            run_with_tile_stats(iterations);
+            return;
+        }
         if (this->params.max_runtime_seconds >= static_cast<std::size_t>(infrastructure::StopWatch::MAX_TIME)) {
             check_gol(gol_step(handle_, iterations));
             check_gol(gol_sync(handle_));
@@ -177,6 +182,40 @@ class GoLHip : public infrastructure::Algorithm<2, grid_cell_t> {
     std::size_t actually_performed_iterations() const override { return performed_; }
 
    private:
+    // --collect-touched-tiles-stats (the reference's cuda_local_one_cell.hpp:60-70,
+    // 130-155, 200-270): one block of temporal_steps generations at a time, then
+    // the tiles it advanced, printed per generation as the reference's three
+    // lines. A tile here is 32 * lane_dwords cells x (64 - 2 * temporal_steps)
+    // rows, advanced for a whole block; a skipped tile needs no copy (its state
+    // from two blocks ago, still in the output buffer, equals its state now), so
+    // Copied_tiles is 0 and Untouched_tiles counts every tile not advanced.
+    void run_with_tile_stats(size_type iterations) {
+        infrastructure::StopWatch watch(this->params.max_runtime_seconds);
+        gol_info info{};
+        check_gol(gol_get_info(handle_, &info));
+        const size_type block = std::max<size_type>(1, info.temporal_steps);
+        const size_type tiles = info.tiles;
+        std::vector<size_type> computed;
+        uint64_t before = info.active_tiles_total;
+        while (performed_ < iterations && !watch.time_is_up()) {
+            const size_type n = std::min(block, iterations - performed_);
+            check_gol(gol_step(handle_, n));
+            check_gol(gol_get_info(handle_, &info));  // waits for the block
+            // a block shorter than temporal_steps advances every tile untracked
+            const size_type advanced = n < block ? tiles : static_cast<size_type>(info.active_tiles_total - before);
+            before = info.active_tiles_total;
+            computed.insert(computed.end(), n, advanced);
+            performed_ += n;
+        }
+        std::cout << "Changed_tiles: ";
+        for (size_type c : computed) std::cout << c << ";";
+        std::cout << std::endl << "Copied_tiles: ";
+        for (size_type i = 0; i < computed.size(); ++i) std::cout << 0 << ";";
+        std::cout << std::endl << "Untouched_tiles: ";
+        for (size_type c : computed) std::cout << tiles - c << ";";
+        std::cout << std::endl;
+    }
+
     void release() {
         if (handle_) {
             gol_destroy(handle_);

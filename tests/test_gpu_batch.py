@@ -243,6 +243,27 @@ def test_persistent_chained_items_of_one_tile_grids():
     assert np.array_equal(results[True][0], ol.bitpar_run(golhip.fill_bernoulli(x, y, seed, p), x, y, g))
 
 
+def test_persistent_long_waits_of_wide_one_tile_grids():
+    """Wide one-tile grids (2048 x 256: not narrow, one workgroup) past 2^20
+    generations: each is two chained items of about 1.6 s each on one CU, so a
+    chained item can wait through another workgroup's whole item; the wait is
+    bounded by device time, not by a poll count. Equal to the per-round
+    launches."""
+    gens = [(1 << 20) + 1000 + 77 * k for k in range(3)]
+    specs = [(2048, 256, g, 0.35, 61 + k) for k, g in enumerate(gens)] + \
+        [(128, 128, 5000 + 100 * k, 0.4, 70 + k) for k in range(4)]
+    results = {}
+    for persistent in (False, True):
+        with golhip.Batch(persistent=persistent) as b:
+            for x, y, g, p, seed in specs:
+                b.add(golhip.fill_bernoulli(x, y, seed, p), x, y, g)
+            note("wide one-tile grids past 2^20 generations, persistent", persistent)
+            b.run()
+            results[persistent] = [b.fetch(i) for i in range(len(specs))]
+    for i, (x, y, g, p, seed) in enumerate(specs):
+        assert np.array_equal(results[True][i], results[False][i]), (x, y, g)
+
+
 def test_persistent_batch_of_zero_generation_grids():
     """No items at all: nothing is launched and every grid keeps its initial state."""
     with golhip.Batch(persistent=True) as b:

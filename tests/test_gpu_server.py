@@ -82,11 +82,19 @@ def test_pinned_region_fills_then_reuses():
     inputs = [golhip.fill_bernoulli(x, y, sd, p) for x, y, g, p, sd in specs]
     with golhip.Server(hbm_budget_bytes=64 << 20) as srv:
         note("pinned region")
+        st = srv.stats()
+        assert st.pinned_region_bytes == 2 << 20 and st.pinned_region_used == 0
         for rnd in range(2):
             ids = [srv.submit(w, x, y, g) for w, (x, y, g, p, sd) in zip(inputs, specs)]
+            st = srv.stats()
+            # the first 32 jobs fill the region, the other 16 pin their own buffers
+            assert st.pinned_region_used == st.pinned_region_bytes == st.pinned_region_peak
+            assert st.pinned_own_jobs == 16 * (rnd + 1)
             for i, (x, y, g, p, sd) in enumerate(specs):
                 out, _ = srv.wait(ids[i])
                 assert np.array_equal(out, alone(inputs[i], x, y, g)), (rnd, i)
+            # every block went back to the region
+            assert srv.stats().pinned_region_used == 0
         assert srv.stats().completed == 2 * len(specs)
 
 
