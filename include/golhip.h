@@ -103,6 +103,15 @@ typedef struct gol_config {
                                        publishes, so each wait ends by its time-out and the launch
                                        stops; the next gol_sync / gol_download reports
                                        GOL_ERR_RUNTIME and the handle is usable after an upload */
+#define GOL_FLAG_NATIVE_LAYOUT 4u /* advance the layout's own words, without converting to rows: 8x8 tiles
+                                     (GOL_LAYOUT_TILES, 64-bit words) or columns (GOL_LAYOUT_COLUMNS, 32- or
+                                     64-bit words), with GOL_KERNEL_PLAIN (one generation per launch;
+                                     reference: cuda_naive_bitwise_kernel.cu:32-79 on BitTile / BitColumns)
+                                     or GOL_KERNEL_TEMPORAL / AUTO (temporal_steps generations per launch,
+                                     1..16, default 16 for tiles and 8 for columns; reference:
+                                     cuda-temporal-simple-tiled/temporal_simple_tiled.cu:41-132). Without
+                                     the flag those layouts are converted to rows on the device, which is
+                                     faster: the row network is the shortest */
 
 typedef struct gol_info {
     uint32_t kernel;          /* resolved enum gol_kernel */
@@ -118,6 +127,8 @@ typedef struct gol_info {
     uint64_t active_tiles_total; /* GOL_KERNEL_ACTIVE: tiles computed, summed over blocks (syncs) */
     uint64_t tracked_blocks;     /* GOL_KERNEL_ACTIVE: blocks of temporal_steps generations run */
     uint64_t device_bytes;    /* device memory owned by the handle */
+    uint32_t native_layout;   /* 1: the layout's own words are advanced (GOL_FLAG_NATIVE_LAYOUT) */
+    uint32_t reserved;
 } gol_info;
 
 typedef struct gol_handle gol_handle;

@@ -178,6 +178,10 @@ struct gol_handle {
     unsigned int* slab_error = nullptr;
     unsigned long long slab_epoch = 0;
     bool slab_check = false;              // a slab launch ran since the error flag was last read
+    // native layout kernels (GOL_FLAG_NATIVE_LAYOUT): the buffers hold the layout's words
+    uint32_t native = 0;  // 0, or the enum gol_layout advanced in its own words
+    Tile8Geo t8{};
+    ColGeo cols{};
 };
 
 namespace {
@@ -205,6 +209,16 @@ int validate_config(const gol_config* c) {
     if (c->lane_dwords != 0 && c->lane_dwords != 1 && c->lane_dwords != 2 && c->lane_dwords != 4 &&
         c->lane_dwords != 8)
         return fail(GOL_ERR_INVALID_ARGUMENT, "lane_dwords must be 0, 1, 2, 4 or 8");
+    if (c->flags & GOL_FLAG_NATIVE_LAYOUT) {
+        const bool tiles = c->layout == GOL_LAYOUT_TILES && c->word_bits == 64;
+        const bool cols = c->layout == GOL_LAYOUT_COLUMNS && (c->word_bits == 32 || c->word_bits == 64);
+        if (!tiles && !cols)
+            return fail(GOL_ERR_INVALID_ARGUMENT, "GOL_FLAG_NATIVE_LAYOUT takes 64-bit 8x8 tiles or 32- / 64-bit "
+                                                  "columns");
+        if (c->kernel == GOL_KERNEL_ACTIVE)
+            return fail(GOL_ERR_INVALID_ARGUMENT, "GOL_FLAG_NATIVE_LAYOUT runs the plain or temporal kernel, not "
+                                                  "change tracking");
+    }
     return GOL_OK;
 }
 
@@ -212,6 +226,25 @@ int validate_config(const gol_config* c) {
 int plan(gol_handle* h) {
     const gol_config& c = h->cfg;
     h->kernel = c.kernel == GOL_KERNEL_AUTO ? GOL_KERNEL_TEMPORAL : c.kernel;
+    if (c.flags & GOL_FLAG_NATIVE_LAYOUT) {
+        // the layout's own words: one generation per launch (plain) or up to
+        // 16; by default 16 for tiles (a halo of 2 tiles) and 8 for columns
+        // (8 halo lanes a side), the fastest of 4..16 on 16384^2
+        const int fallback = c.layout == GOL_LAYOUT_TILES ? 16 : 8;
+        const int steps =
+            h->kernel == GOL_KERNEL_PLAIN ? 1 : (c.temporal_steps ? static_cast<int>(c.temporal_steps) : fallback);
+        const char* why = nullptr;
+        h->native = c.layout;
+        h->steps = steps;
+        const bool ok = c.layout == GOL_LAYOUT_TILES
+                            ? plan_tile8(static_cast<int64_t>(c.x_cells / 8), static_cast<int64_t>(c.y_cells / 8),
+                                         steps, &h->t8, &why)
+                            : plan_columns(static_cast<int64_t>(c.x_cells), static_cast<int64_t>(c.y_cells),
+                                           static_cast<int>(c.word_bits), steps, &h->cols, &why);
+        if (!ok) return fail(GOL_ERR_INVALID_ARGUMENT, "temporal_steps %d: %s", steps, why);
+        h->n_tiles = c.layout == GOL_LAYOUT_TILES ? h->t8.groups_x * h->t8.groups_y : h->cols.groups_x * h->cols.groups_v;
+        return GOL_OK;
+    }
     if (h->kernel == GOL_KERNEL_PLAIN) {
         uint32_t dw = c.lane_dwords;
         if (dw == 0) dw = h->x_dw <= 64 ? 1 : (h->x_dw <= 128 ? 2 : 4);
@@ -393,6 +426,20 @@ int enqueue_active_block(gol_handle* h) {
 int enqueue(gol_handle* h, uint64_t generations) {
     if (h->persistent) return enqueue_persistent(h, generations);
     uint64_t left = generations;
+    if (h->native) {
+        while (left > 0) {
+            const int gens = static_cast<int>(std::min<uint64_t>(left, static_cast<uint64_t>(h->steps)));
+            const hipError_t e = h->native == GOL_LAYOUT_TILES
+                                     ? launch_tile8(h->t8, h->buf[h->cur], h->buf[h->cur ^ 1], gens, h->stream)
+                                     : launch_columns(h->cols, h->buf[h->cur], h->buf[h->cur ^ 1], gens, h->stream);
+            if (e != hipSuccess) return fail_hip(e, "native layout kernel launch");
+            h->cur ^= 1;
+            h->launches += 1;
+            h->generations += static_cast<uint64_t>(gens);
+            left -= static_cast<uint64_t>(gens);
+        }
+        return GOL_OK;
+    }
     if (h->kernel == GOL_KERNEL_ACTIVE) {
         while (left >= static_cast<uint64_t>(h->steps)) {
             int rc = enqueue_active_block(h);
@@ -476,7 +523,7 @@ int gol_create(const gol_config* config, gol_handle** out) {
         (e = hipEventCreate(&h->ev_start)) != hipSuccess || (e = hipEventCreate(&h->ev_stop)) != hipSuccess ||
         (e = hipMalloc(&h->buf[0], bytes)) != hipSuccess || (e = hipMalloc(&h->buf[1], bytes)) != hipSuccess ||
         (e = hipMemsetAsync(h->buf[0], 0, bytes, h->stream)) != hipSuccess ||
-        (h->cfg.layout != GOL_LAYOUT_ROWS && (e = hipMalloc(&h->staging, bytes)) != hipSuccess)) {
+        (h->cfg.layout != GOL_LAYOUT_ROWS && !h->native && (e = hipMalloc(&h->staging, bytes)) != hipSuccess)) {
         rc = fail_hip(e, "gol_create");
         gol_destroy(h);
         return rc;
@@ -523,7 +570,7 @@ int gol_upload(gol_handle* h, const void* words, size_t n_words) {
     if (n_words != want) return fail(GOL_ERR_INVALID_ARGUMENT, "expected %zu words, got %zu", want, n_words);
     DeviceGuard g(h->device);
     hipError_t e;
-    if (h->cfg.layout != GOL_LAYOUT_ROWS) {
+    if (h->cfg.layout != GOL_LAYOUT_ROWS && !h->native) {
         if ((e = hipMemcpyAsync(h->staging, words, h->n_dwords * 4, hipMemcpyHostToDevice, h->stream)) == hipSuccess)
             e = layout_to_rows(static_cast<int>(h->cfg.layout), static_cast<int>(h->cfg.word_bits), h->staging,
                                h->buf[h->cur], h->x_dw * 32, h->y_rows, h->stream);
@@ -543,7 +590,7 @@ int gol_upload_device(gol_handle* h, const void* device_words, size_t n_words) {
     if (n_words != want) return fail(GOL_ERR_INVALID_ARGUMENT, "expected %zu words, got %zu", want, n_words);
     DeviceGuard g(h->device);
     hipError_t e =
-        h->cfg.layout != GOL_LAYOUT_ROWS
+        h->cfg.layout != GOL_LAYOUT_ROWS && !h->native
             ? layout_to_rows(static_cast<int>(h->cfg.layout), static_cast<int>(h->cfg.word_bits), device_words,
                              h->buf[h->cur], h->x_dw * 32, h->y_rows, h->stream)
             : hipMemcpyAsync(h->buf[h->cur], device_words, h->n_dwords * 4, hipMemcpyDeviceToDevice, h->stream);
@@ -587,7 +634,7 @@ int gol_download(gol_handle* h, void* words, size_t n_words) {
     if (rc) return rc;
     hipError_t e = hipSuccess;
     const void* from = h->buf[h->cur];
-    if (h->cfg.layout != GOL_LAYOUT_ROWS) {
+    if (h->cfg.layout != GOL_LAYOUT_ROWS && !h->native) {
         e = rows_to_layout(static_cast<int>(h->cfg.layout), static_cast<int>(h->cfg.word_bits), h->buf[h->cur],
                            h->staging, h->x_dw * 32, h->y_rows, h->stream);
         from = h->staging;
@@ -626,7 +673,13 @@ int gol_get_info(gol_handle* h, gol_info* info) {
     if (!h || !info) return fail(GOL_ERR_INVALID_ARGUMENT, "null argument");
     std::memset(info, 0, sizeof(*info));
     info->kernel = h->kernel;
-    if (h->kernel == GOL_KERNEL_PLAIN) {
+    if (h->native) {
+        // workgroups (tiles) and wavefronts of one launch
+        info->temporal_steps = static_cast<uint32_t>(h->steps);
+        info->tiles = static_cast<uint32_t>(h->n_tiles);
+        info->waves = static_cast<uint32_t>(h->native == GOL_LAYOUT_TILES ? kTile8Waves : 1);
+        info->band_rows = static_cast<uint32_t>(h->native == GOL_LAYOUT_TILES ? kTile8Rows : kColDwords);
+    } else if (h->kernel == GOL_KERNEL_PLAIN) {
         info->lane_dwords = h->plain_dw;
         info->temporal_steps = 1;
     } else {
@@ -642,6 +695,7 @@ int gol_get_info(gol_handle* h, gol_info* info) {
     info->generations = h->generations;
     info->device_bytes = (h->staging ? 3 : 2) * h->n_dwords * 4;
     info->layout = h->cfg.layout;
+    info->native_layout = h->native ? 1u : 0u;
     if (h->kernel == GOL_KERNEL_ACTIVE) {
         DeviceGuard g(h->device);
         std::vector<unsigned long long> per_wg(static_cast<size_t>(fine_workgroups(h->fine)));

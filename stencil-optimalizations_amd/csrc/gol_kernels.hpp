@@ -199,6 +199,42 @@ hipError_t layout_to_rows(int layout, int word_bits, const void* src, uint32_t* 
 hipError_t rows_to_layout(int layout, int word_bits, const uint32_t* rows, void* dst, int64_t x_cells,
                           int64_t y_rows, hipStream_t stream);
 
+// Native kernels of the reference's 8x8-tile and column layouts
+// (GOL_FLAG_NATIVE_LAYOUT): the grid stays in the layout's words.
+//   8x8 tiles, 64-bit words: a workgroup of kTile8Waves wavefronts, each lane
+//   one tile column kTile8Rows tiles tall, so 64 tile columns x
+//   kTile8Rows * kTile8Waves tile rows per workgroup, with a halo of
+//   ceil(T / 8) tiles around it; X a multiple of 32 cells (4 tiles), Y of 8.
+constexpr int kTile8Rows = 8;
+constexpr int kTile8Waves = 8;
+constexpr int kTile8MaxSteps = 16;
+struct Tile8Geo {
+    int x_tiles, y_tiles;    // grid size in tiles
+    int halo_x, halo_y;      // halo tiles left and right / above and below (0 where one tile spans the grid)
+    int stride_x, stride_y;  // interior tiles per workgroup
+    int groups_x, groups_y;  // workgroups across and down
+};
+bool plan_tile8(int64_t x_tiles, int64_t y_tiles, int gens, Tile8Geo* g, const char** why);
+// Advance gens (<= the planned) generations from src into dst (tile words).
+hipError_t launch_tile8(const Tile8Geo& g, const uint32_t* src, uint32_t* dst, int gens, hipStream_t stream);
+
+//   columns, 32- or 64-bit words: each wavefront its own tile of 64 grid
+//   columns (one per lane) x kColDwords dwords of 32 rows, with T lanes of
+//   halo left and right and one dword (32 rows) above and below; Y a multiple
+//   of the word.
+constexpr int kColDwords = 34;
+constexpr int kColWaves = 4;
+constexpr int kColMaxSteps = 16;
+struct ColGeo {
+    int x_cells, v_dwords;   // grid columns; dwords of 32 rows per column (Y / 32)
+    int word_bits;           // 32 or 64
+    int halo_x, halo_v;      // halo lanes left and right / dwords above and below
+    int stride_x, stride_v;  // interior columns / dwords per tile
+    int groups_x, groups_v;  // tiles across and down
+};
+bool plan_columns(int64_t x_cells, int64_t y_cells, int word_bits, int gens, ColGeo* g, const char** why);
+hipError_t launch_columns(const ColGeo& g, const uint32_t* src, uint32_t* dst, int gens, hipStream_t stream);
+
 // Plain kernel: one generation per launch, global memory to global memory.
 // lane_dwords in {1,2,4,8}.
 hipError_t launch_plain(int lane_dwords, const uint32_t* src, uint32_t* dst, int64_t x_dw, int64_t y_rows,

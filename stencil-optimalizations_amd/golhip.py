@@ -74,6 +74,7 @@ class GolConfig(ctypes.Structure):
 
 FLAG_NO_PERSISTENT = 1
 FLAG_SLAB_WAIT_PROBE = 2
+FLAG_NATIVE_LAYOUT = 4
 # gol_batch_config.flags
 BATCH_PERSISTENT = 1
 BATCH_WAIT_PROBE = 2
@@ -97,6 +98,8 @@ class GolInfo(ctypes.Structure):
         ("active_tiles_total", ctypes.c_uint64),
         ("tracked_blocks", ctypes.c_uint64),
         ("device_bytes", ctypes.c_uint64),
+        ("native_layout", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 

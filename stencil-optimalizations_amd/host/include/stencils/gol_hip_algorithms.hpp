@@ -11,7 +11,12 @@
 //                               (the role of cuda-naive-local-one-cell, cuda_local_one_cell.cu:53-242)
 //   gol-hip-auto-64             shape and depth chosen by the library's time model
 //   gol-hip-cols-16/32/64       the same engine fed in the reference's column
-//   gol-hip-tiles-16/32/64      and tile bit layouts (BitColumns, BitTile)
+//   gol-hip-tiles-16/32/64      and tile bit layouts (BitColumns, BitTile), converted to rows
+//   gol-hip-tiles-plain-64      kernels native to the 8x8-tile layout: one generation per launch
+//   gol-hip-tiles-temporal-64   (cuda_naive_bitwise_kernel.cu:32-79 on BitTile, tiled-full-adder.cuh)
+//                               and T per launch (temporal_simple_tiled.cu:41-132)
+//   gol-hip-cols-plain-32/64    kernels native to the column layout, likewise
+//   gol-hip-cols-temporal-32/64
 //
 // Parameters read from ExperimentParams: temporal_steps (> 1: generations per
 // launch; 0 or 1: chosen by the library), thread_block_size (> 0: wavefronts
@@ -116,7 +121,8 @@ struct LayoutWords {
 template <std::size_t Bits>
 using RowWords = LayoutWords<GOL_LAYOUT_ROWS, Bits>;
 
-template <typename grid_cell_t, std::size_t Bits, uint32_t Kernel, uint32_t Layout = GOL_LAYOUT_ROWS>
+template <typename grid_cell_t, std::size_t Bits, uint32_t Kernel, uint32_t Layout = GOL_LAYOUT_ROWS,
+          uint32_t Flags = 0>
 class GoLHip : public infrastructure::Algorithm<2, grid_cell_t> {
    public:
     using size_type = std::size_t;
@@ -143,6 +149,7 @@ class GoLHip : public infrastructure::Algorithm<2, grid_cell_t> {
         c.temporal_steps = this->params.temporal_steps > 1 ? static_cast<uint32_t>(this->params.temporal_steps) : 0;
         c.waves = this->params.thread_block_size >= 64 ? static_cast<uint32_t>(this->params.thread_block_size / 64) : 0;
         c.device = -1;
+        c.flags = Flags;
         check_gol(gol_create(&c, &handle_));
         check_gol(gol_upload(handle_, words_.data(), words_.size()));
     }
@@ -247,6 +254,22 @@ void register_hip_algorithms(infrastructure::AlgorithmRepository<2, cell_t>& rep
     repo.template register_algorithm<GoLHip<cell_t, 16, GOL_KERNEL_AUTO, GOL_LAYOUT_TILES>>("gol-hip-tiles-16");
     repo.template register_algorithm<GoLHip<cell_t, 32, GOL_KERNEL_AUTO, GOL_LAYOUT_TILES>>("gol-hip-tiles-32");
     repo.template register_algorithm<GoLHip<cell_t, 64, GOL_KERNEL_AUTO, GOL_LAYOUT_TILES>>("gol-hip-tiles-64");
+    // kernels native to those layouts (no conversion): the 8x8-tile adder one
+    // generation per launch (the reference's gol-cuda-naive-bitwise-tiles-64,
+    // "Core Tiled 64-bit") and T per launch (cuda-temporal-simple-tiled), and
+    // the column kernel likewise (gol-cuda-naive-bitwise-cols-32/64)
+    repo.template register_algorithm<GoLHip<cell_t, 64, GOL_KERNEL_PLAIN, GOL_LAYOUT_TILES, GOL_FLAG_NATIVE_LAYOUT>>(
+        "gol-hip-tiles-plain-64");
+    repo.template register_algorithm<GoLHip<cell_t, 64, GOL_KERNEL_TEMPORAL, GOL_LAYOUT_TILES, GOL_FLAG_NATIVE_LAYOUT>>(
+        "gol-hip-tiles-temporal-64");
+    repo.template register_algorithm<GoLHip<cell_t, 32, GOL_KERNEL_PLAIN, GOL_LAYOUT_COLUMNS, GOL_FLAG_NATIVE_LAYOUT>>(
+        "gol-hip-cols-plain-32");
+    repo.template register_algorithm<GoLHip<cell_t, 64, GOL_KERNEL_PLAIN, GOL_LAYOUT_COLUMNS, GOL_FLAG_NATIVE_LAYOUT>>(
+        "gol-hip-cols-plain-64");
+    repo.template register_algorithm<GoLHip<cell_t, 32, GOL_KERNEL_TEMPORAL, GOL_LAYOUT_COLUMNS, GOL_FLAG_NATIVE_LAYOUT>>(
+        "gol-hip-cols-temporal-32");
+    repo.template register_algorithm<GoLHip<cell_t, 64, GOL_KERNEL_TEMPORAL, GOL_LAYOUT_COLUMNS, GOL_FLAG_NATIVE_LAYOUT>>(
+        "gol-hip-cols-temporal-64");
 }
 
 }  // namespace algorithms

@@ -205,7 +205,9 @@ static void gpu_tests_for_encoding() {
     }
     // the column and tile layouts: heights a multiple of 64 (a 64-bit column word)
     for (const std::string name : {"gol-hip-cols-16", "gol-hip-cols-32", "gol-hip-cols-64", "gol-hip-tiles-16",
-                                   "gol-hip-tiles-32", "gol-hip-tiles-64"}) {
+                                   "gol-hip-tiles-32", "gol-hip-tiles-64", "gol-hip-tiles-plain-64",
+                                   "gol-hip-tiles-temporal-64", "gol-hip-cols-plain-32", "gol-hip-cols-plain-64",
+                                   "gol-hip-cols-temporal-32", "gol-hip-cols-temporal-64"}) {
         std::cout << "[case] plug-in " << name << " cell bytes " << sizeof(cell_t) << std::endl;
         auto* alg = repo.fetch_algorithm(name);
         CHECK(alg != nullptr);

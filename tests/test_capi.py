@@ -72,6 +72,19 @@ def test_layout_size_errors_before_device(x, y, w, layout, multiple):
     assert multiple in str(e.value)
 
 
+@pytest.mark.parametrize("kw", [
+    dict(x_cells=64, y_cells=64, word_bits=64),                                     # rows
+    dict(x_cells=64, y_cells=64, word_bits=32, layout=golhip.LAYOUT_TILES),         # 8x4 tiles
+    dict(x_cells=64, y_cells=64, word_bits=16, layout=golhip.LAYOUT_COLUMNS),       # 16-bit columns
+    dict(x_cells=64, y_cells=64, word_bits=64, layout=golhip.LAYOUT_TILES, kernel=golhip.KERNEL_ACTIVE),
+])
+def test_native_layout_flag_errors_before_device(kw):
+    with pytest.raises(golhip.GolError) as e:
+        golhip.Grid(flags=golhip.FLAG_NATIVE_LAYOUT, **kw)
+    assert e.value.code == golhip.GOL_ERR_INVALID_ARGUMENT
+    assert "GOL_FLAG_NATIVE_LAYOUT" in str(e.value)
+
+
 def test_error_message_names_the_multiple():
     with pytest.raises(golhip.GolError) as e:
         golhip.Grid(96, 8, word_bits=64)
