@@ -128,7 +128,7 @@ typedef struct gol_info {
     uint64_t tracked_blocks;     /* GOL_KERNEL_ACTIVE: blocks of temporal_steps generations run */
     uint64_t device_bytes;    /* device memory owned by the handle */
     uint32_t native_layout;   /* 1: the layout's own words are advanced (GOL_FLAG_NATIVE_LAYOUT) */
-    uint32_t reserved;
+    uint32_t workgroups;      /* GOL_KERNEL_ACTIVE: workgroups per launch (one wavefront per tile) */
 } gol_info;
 
 typedef struct gol_handle gol_handle;

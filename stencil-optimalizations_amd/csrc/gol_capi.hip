@@ -165,8 +165,8 @@ struct gol_handle {
     // change tracking (GOL_KERNEL_ACTIVE): small tiles, one wavefront each
     FineGeo fine{};
     uint32_t* marks = nullptr;    // [3][n_tiles] tile is on list k (lists rotate by block)
-    int* lists = nullptr;         // [3][n_tiles] active-tile lists
-    int* counts = nullptr;        // [3] list lengths
+    int* lists = nullptr;         // [3][kFineLists][fine_list_cap] active-tile lists
+    int* counts = nullptr;        // [3][kFineLists][kFineCountStride] sub-list lengths
     unsigned long long* wg_active = nullptr;  // [fine_workgroups] tiles advanced, per workgroup
     int n_tiles = 0;
     uint64_t blocks = 0;          // change-tracking blocks run
@@ -281,7 +281,7 @@ int plan(gol_handle* h) {
         // holds at once
         const int all = (h->n_tiles + kFineWaves - 1) / kFineWaves;
         const int resident = fine_resident_workgroups(dw, steps, h->x_dw);
-        h->fine.workgroups = resident > 0 ? std::min(all, resident) : all;
+        h->fine.workgroups = std::max(resident > 0 ? std::min(all, resident) : all, kFineMinWorkgroups);
         h->shape = TileShape{dw, 1, kFineWaves};
         return GOL_OK;
     }
@@ -401,19 +401,20 @@ int enqueue_active_block(gol_handle* h) {
     if (!h->flags_valid) {
         // a new state: every tile advances, and no list or mark is pending
         if ((e = hipMemsetAsync(h->marks, 0, 3 * n * sizeof(uint32_t), h->stream)) != hipSuccess ||
-            (e = hipMemsetAsync(h->counts, 0, 3 * sizeof(int), h->stream)) != hipSuccess)
+            (e = hipMemsetAsync(h->counts, 0, 3 * kFineCounts * sizeof(int), h->stream)) != hipSuccess)
             return fail_hip(e, "hipMemsetAsync");
     }
-    const int k = static_cast<int>(h->blocks % 3), k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+    const size_t k = static_cast<size_t>(h->blocks % 3), k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+    const size_t list_len = kFineLists * static_cast<size_t>(fine_list_cap(h->fine));
     FineTrack tr;
     tr.mark_cur = h->marks + k * n;
     tr.mark_next = h->marks + k1 * n;
-    tr.list_next = h->lists + k1 * n;
-    tr.count_next = h->counts + k1;
-    tr.count_clear = h->counts + k2;
+    tr.list_next = h->lists + k1 * list_len;
+    tr.count_next = h->counts + k1 * kFineCounts;
+    tr.count_clear = h->counts + k2 * kFineCounts;
     tr.wg_active = h->wg_active;
-    e = launch_fine_tiles(h->fine, h->buf[h->cur], h->buf[h->cur ^ 1], h->steps,
-                          h->flags_valid ? h->lists + k * n : nullptr, h->counts + k, tr, h->stream);
+    e = launch_fine_tiles(h->fine, h->buf[h->cur], h->buf[h->cur ^ 1], h->steps, h->lists + k * list_len,
+                          h->counts + k * kFineCounts, h->flags_valid, tr, h->stream);
     if (e != hipSuccess) return fail_hip(e, "fine_tile_kernel launch");
     h->flags_valid = true;
     h->cur ^= 1;
@@ -460,7 +461,8 @@ int enqueue(gol_handle* h, uint64_t generations) {
             done = 1;
         } else if (h->kernel == GOL_KERNEL_ACTIVE) {
             done = std::min<uint64_t>(left, static_cast<uint64_t>(h->steps));
-            e = launch_fine_tiles(h->fine, src, dst, static_cast<int>(done), nullptr, nullptr, FineTrack{}, h->stream);
+            e = launch_fine_tiles(h->fine, src, dst, static_cast<int>(done), h->lists, h->counts, false, FineTrack{},
+                                  h->stream);
         } else {
             done = std::min<uint64_t>(left, static_cast<uint64_t>(h->steps));
             e = launch_tiles(h->geom, src, dst, h->x_dw, h->y_rows, static_cast<int>(done), h->stream);
@@ -531,9 +533,10 @@ int gol_create(const gol_config* config, gol_handle** out) {
     if (h->kernel == GOL_KERNEL_ACTIVE) {
         const size_t nt = static_cast<size_t>(h->n_tiles);
         const size_t nwg = static_cast<size_t>(fine_workgroups(h->fine));
+        const size_t list_len = kFineLists * static_cast<size_t>(fine_list_cap(h->fine));
         if ((e = hipMalloc(&h->marks, 3 * nt * sizeof(uint32_t))) != hipSuccess ||
-            (e = hipMalloc(&h->lists, 3 * nt * sizeof(int))) != hipSuccess ||
-            (e = hipMalloc(&h->counts, 3 * sizeof(int))) != hipSuccess ||
+            (e = hipMalloc(&h->lists, 3 * list_len * sizeof(int))) != hipSuccess ||
+            (e = hipMalloc(&h->counts, 3 * kFineCounts * sizeof(int))) != hipSuccess ||
             (e = hipMalloc(&h->wg_active, nwg * sizeof(unsigned long long))) != hipSuccess ||
             (e = hipMemsetAsync(h->wg_active, 0, nwg * sizeof(unsigned long long), h->stream)) != hipSuccess) {
             rc = fail_hip(e, "gol_create (change tracking state)");
@@ -707,8 +710,10 @@ int gol_get_info(gol_handle* h, gol_info* info) {
         for (unsigned long long v : per_wg) total += v;
         info->active_tiles_total = total;
         info->tracked_blocks = h->blocks;
-        info->device_bytes += static_cast<uint64_t>(h->n_tiles) * 3 * (sizeof(uint32_t) + sizeof(int)) +
-                              3 * sizeof(int) + per_wg.size() * sizeof(unsigned long long);
+        info->workgroups = static_cast<uint32_t>(per_wg.size());
+        info->device_bytes += static_cast<uint64_t>(h->n_tiles) * 3 * sizeof(uint32_t) +
+                              3ull * kFineLists * fine_list_cap(h->fine) * sizeof(int) +
+                              3ull * kFineCounts * sizeof(int) + per_wg.size() * sizeof(unsigned long long);
     }
     return GOL_OK;
 }

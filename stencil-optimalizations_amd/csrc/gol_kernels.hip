@@ -393,9 +393,11 @@ __device__ __forceinline__ void fine_load(const uint32_t* __restrict__ src, cons
 // Advances one tile, whose words fine_load read, by gens generations. With
 // tracking (tr.list_next set), a tile whose cells changed appends the tiles of
 // its 3 x 3 neighbourhood to the next block's list: a tile joins once, when
-// its mark goes from 0 to 1, and the wavefront reserves its entries with one
-// atomic add. The wavefront clears the mark of a listed tile it advances (its
-// list is done with it).
+// its mark goes from 0 to 1, to sub-list fine_sub_list(nx, ny) of the new
+// tile, whose counter reserves its entry. Sub-lists spread the reservations,
+// device-scope atomics that one counter would take one after another, over
+// kFineLists counters in lines of their own. The wavefront clears the mark
+// of a listed tile it advances (its list is done with it).
 template <int DW, bool RING, bool VEC>
 __device__ __forceinline__ void fine_tile(const FineRow<DW>& r, uint32_t* __restrict__ dst, const FineGeo& g,
                                           int gens, int tile, bool listed, const FineTrack& tr) {
@@ -462,53 +464,65 @@ __device__ __forceinline__ void fine_tile(const FineRow<DW>& r, uint32_t* __rest
     }
     if (__ballot(own && diff) == 0) return;
     // lanes 0..8: the 3 x 3 neighbourhood joins the next block's list
-    int n = 0;
-    bool add = false;
     if (lane < 9) {
         const int nx = tx + lane % 3 - 1, ny = ty + lane / 3 - 1;
         if (nx >= 0 && nx < g.tiles_x && ny >= 0 && ny < g.tiles_y) {
-            n = ny * g.tiles_x + nx;
-            add = atomicOr(&tr.mark_next[n], 1u) == 0u;
+            const int n = ny * g.tiles_x + nx;
+            if (atomicOr(&tr.mark_next[n], 1u) == 0u) {
+                const int j = fine_sub_list(nx, ny);
+                const int at = atomicAdd(tr.count_next + j * kFineCountStride, 1);
+                tr.list_next[static_cast<size_t>(j) * tr.list_cap + at] = n;
+            }
         }
     }
-    const unsigned long long m = __ballot(add);
-    if (m == 0) return;
-    int base = 0;
-    if (lane == 0) base = atomicAdd(tr.count_next, __popcll(m));
-    base = __shfl(base, 0);
-    if (add) tr.list_next[base + __popcll(m & ((1ull << lane) - 1))] = n;
 }
 
-// Wavefronts take slots i, i + stride, ... of tile list[0 .. *count), or of
-// tiles 0 .. dense_tiles - 1 when list is null, and read one tile ahead: the
-// loads of the next tile are in flight while the current tile advances (a
-// block lists more tiles than the launch has wavefronts when much of the grid
-// is active). The list entry after it is read before the tile advances, a
-// scalar load the wavefront waits for. The launch resets the counter of the
-// list after next.
+// A tracked block advances the tiles of kFineLists sub-lists: wavefront w
+// (of W = kFineWaves * gridDim.x, at least kFineLists) takes sub-list
+// w % kFineLists, entries r, r + step, ... with r = w / kFineLists and step
+// the number of wavefronts on that sub-list. An untracked block advances
+// tiles w, w + W, ... < dense_tiles. Either way a wavefront reads one tile
+// ahead: the loads of the next tile are in flight while the current tile
+// advances (a block lists more tiles than the launch has wavefronts when much
+// of the grid is active). The entry after it is read before the tile
+// advances, a scalar load the wavefront waits for. The launch resets the
+// counters of the list after next.
 template <int DW, bool RING, bool VEC>
 __global__ __launch_bounds__(64 * kFineWaves) void fine_tile_kernel(const uint32_t* __restrict__ src,
                                                                     uint32_t* __restrict__ dst, FineGeo g, int gens,
                                                                     const int* __restrict__ list,
-                                                                    const int* __restrict__ count, int dense_tiles,
-                                                                    FineTrack tr) {
-    if (tr.count_clear && blockIdx.x == 0 && threadIdx.x == 0) *tr.count_clear = 0;
-    const int n = list ? *count : dense_tiles;
-    const int stride = static_cast<int>(gridDim.x) * kFineWaves;
-    // Consecutive list entries are neighbouring tiles (a wavefront appends its
-    // 3 x 3 neighbourhood at once), which share halo rows. Workgroup b runs on
-    // XCD b % 8, so runs of 128 consecutive slots go to 32 workgroups of one
-    // XCD (one L2), the runs dealt to the XCDs in turn, so that a short list
-    // still spreads over all of them. A bijection onto the workgroups when
-    // their number is a multiple of 256; else workgroup b takes slots
-    // 4b .. 4b + 3.
+                                                                    const int* __restrict__ count, int listed,
+                                                                    int dense_tiles, FineTrack tr) {
+    if (tr.count_clear && blockIdx.x == 0 && threadIdx.x < kFineLists)
+        tr.count_clear[threadIdx.x * kFineCountStride] = 0;
+    const int waves = static_cast<int>(gridDim.x) * kFineWaves;
+    // Workgroup b runs on XCD b % 8; runs of 128 consecutive wavefront
+    // numbers go to 32 workgroups of one XCD (one L2), the runs dealt to the
+    // XCDs in turn. A bijection onto the workgroups when their number is a
+    // multiple of 256; else workgroup b takes numbers 4b .. 4b + 3.
     const int b = static_cast<int>(blockIdx.x);
     const int group = (gridDim.x & 255) == 0 ? ((b >> 8) * 8 + (b & 7)) * 32 + ((b >> 3) & 31) : b;
-    int slot = group * kFineWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (slot >= n) return;
-    auto tile_at = [&](int s) { return list ? __builtin_amdgcn_readfirstlane(list[s]) : s; };
-    int tile = tile_at(slot);
-    int next_tile = slot + stride < n ? tile_at(slot + stride) : -1;
+    const int w = group * kFineWaves + __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    // sub-list, first entry and step of a tracked block; for an untracked one
+    // the "sub-list" is the whole range of tiles
+    const int sub = w % kFineLists;
+    const int cap = listed ? tr.list_cap : dense_tiles;
+    const int step = listed ? (waves - sub + kFineLists - 1) / kFineLists : waves;
+    int at = listed ? w / kFineLists : w;
+    const int* entries = list + static_cast<size_t>(sub) * tr.list_cap;
+    auto tile_at = [&](int i) { return listed ? __builtin_amdgcn_readfirstlane(entries[i]) : i; };
+    // The first entry is read together with the sub-list's length, both loads
+    // unconditional (list and count are valid also for a block that advances
+    // every tile), the entry's index clamped into the sub-list: the wavefront
+    // waits once, for both (the empty asm takes both values, so nothing that
+    // uses one is placed between the loads).
+    int first = __builtin_amdgcn_readfirstlane(entries[at < tr.list_cap ? at : tr.list_cap - 1]);
+    int length = __builtin_amdgcn_readfirstlane(count[sub * kFineCountStride]);
+    asm volatile("" : "+s"(first), "+s"(length));
+    const int n = listed ? length : cap;
+    if (at >= n) return;
+    int tile = listed ? first : at;
+    int next_tile = at + step < n ? tile_at(at + step) : -1;
     FineRow<DW> cur;
     fine_load<DW, VEC>(src, g, tile, cur);
     for (;;) {
@@ -520,15 +534,15 @@ __global__ __launch_bounds__(64 * kFineWaves) void fine_tile_kernel(const uint32
         int after = -1;
         if (next_tile >= 0) {
             fine_load<DW, VEC>(src, g, next_tile, nxt);
-            if (slot + 2 * stride < n) after = tile_at(slot + 2 * stride);
+            if (at + 2 * step < n) after = tile_at(at + 2 * step);
         }
         // keeps the loads here, issued before the tile advances (the compiler
         // would otherwise sink them to their use); it reads no loaded word, so
         // nothing waits for them
         asm volatile("" ::: "memory");
-        fine_tile<DW, RING, VEC>(cur, dst, g, gens, tile, list != nullptr, tr);
+        fine_tile<DW, RING, VEC>(cur, dst, g, gens, tile, listed != 0, tr);
         if (next_tile < 0) return;
-        slot += stride;
+        at += step;
         tile = next_tile;
         next_tile = after;
         cur = nxt;
@@ -1058,22 +1072,25 @@ static bool fine_ring(int halo_rows) { return halo_rows <= kFineRingMaxSteps; }
 static bool fine_vec(int64_t x_dw, int tile_dwords) { return x_dw % (tile_dwords >= 4 ? 4 : tile_dwords) == 0; }
 
 hipError_t launch_fine_tiles(const FineGeo& g, const uint32_t* src, uint32_t* dst, int gens, const int* list,
-                             const int* count, const FineTrack& track, hipStream_t stream) {
+                             const int* count, bool listed, const FineTrack& track, hipStream_t stream) {
     if ((g.tile_dwords != 2 && g.tile_dwords != 4 && g.tile_dwords != 8) || gens > g.halo_rows ||
-        g.halo_rows > kFineMaxSteps)
+        g.halo_rows > kFineMaxSteps || !list || !count)
         return hipErrorInvalidValue;
     const int n = g.tiles_x * g.tiles_y;
+    if (fine_workgroups(g) < kFineMinWorkgroups) return hipErrorInvalidValue;
     const dim3 grid(static_cast<unsigned>(fine_workgroups(g))), block(64 * kFineWaves);
+    FineTrack t = track;
+    t.list_cap = fine_list_cap(g);
     const bool ring = fine_ring(g.halo_rows), vec = fine_vec(g.x_dw, g.tile_dwords);
     if (g.tile_dwords == 2)
-        hipLaunchKernelGGL(fine_kernel_for<2>(ring, vec), grid, block, 0, stream, src, dst, g, gens, list, count, n,
-                           track);
+        hipLaunchKernelGGL(fine_kernel_for<2>(ring, vec), grid, block, 0, stream, src, dst, g, gens, list, count,
+                           listed ? 1 : 0, n, t);
     else if (g.tile_dwords == 4)
-        hipLaunchKernelGGL(fine_kernel_for<4>(ring, vec), grid, block, 0, stream, src, dst, g, gens, list, count, n,
-                           track);
+        hipLaunchKernelGGL(fine_kernel_for<4>(ring, vec), grid, block, 0, stream, src, dst, g, gens, list, count,
+                           listed ? 1 : 0, n, t);
     else
-        hipLaunchKernelGGL(fine_kernel_for<8>(ring, vec), grid, block, 0, stream, src, dst, g, gens, list, count, n,
-                           track);
+        hipLaunchKernelGGL(fine_kernel_for<8>(ring, vec), grid, block, 0, stream, src, dst, g, gens, list, count,
+                           listed ? 1 : 0, n, t);
     return hipGetLastError();
 }
 

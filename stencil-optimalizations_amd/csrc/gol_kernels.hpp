@@ -122,21 +122,40 @@ inline int fine_workgroups(const FineGeo& g) { return g.workgroups; }
 // the current device (0 if unknown).
 int fine_resident_workgroups(int tile_dwords, int halo_rows, int64_t x_dw);
 
+// A block's list of tiles is kFineLists sub-lists: tile (tx, ty) goes to
+// sub-list (tx + 5 ty) mod kFineLists, so that a strip of tiles along a row,
+// a column or a diagonal spreads over all of them. Each row of tiles puts at
+// most ceil(tiles_x / kFineLists) tiles into one sub-list, so a sub-list
+// holds at most list_cap = tiles_y * ceil(tiles_x / kFineLists) entries. The
+// length of sub-list j is counts[j * kFineCountStride] (each counter in a
+// 128-byte line of its own).
+constexpr int kFineLists = 8;
+constexpr int kFineCountStride = 32;
+constexpr int kFineCounts = kFineLists * kFineCountStride;  // ints of one list's counters
+__host__ __device__ inline int fine_sub_list(int tx, int ty) { return (tx + 5 * ty) & (kFineLists - 1); }
+inline int fine_list_cap(const FineGeo& g) { return g.tiles_y * ((g.tiles_x + kFineLists - 1) / kFineLists); }
+// Workgroups a tracked launch needs at least: one wavefront per sub-list.
+constexpr int kFineMinWorkgroups = kFineLists / kFineWaves;
+
 // Change-tracking state of one block (all null: no tracking).
 struct FineTrack {
     uint32_t* mark_cur;             // marks of the list being advanced: cleared as its tiles are taken
     uint32_t* mark_next;            // marks of the next block's list (all 0 when the block starts)
-    int* list_next;                 // next block's list, built by appending
-    int* count_next;                // its length (0 when the block starts)
-    int* count_clear;               // length of the list after next: reset to 0
+    int* list_next;                 // next block's list (kFineLists sub-lists), built by appending
+    int* count_next;                // its sub-lists' lengths (0 when the block starts)
+    int* count_clear;               // lengths of the list after next: reset to 0
     unsigned long long* wg_active;  // tiles advanced, per workgroup
+    int list_cap;                   // entries per sub-list: set by launch_fine_tiles (fine_list_cap)
 };
 
-// One change-tracking block: advance tiles list[0 .. *count), or every tile
-// when list is null, by gens (<= halo_rows) generations; with track set, the
-// tiles whose cells changed append their 3 x 3 neighbourhoods to the next list.
+// One change-tracking block: advance the tiles of list's sub-lists (lengths
+// in count) when listed, else every tile, by gens (<= halo_rows)
+// generations; with track set, the tiles whose cells changed append their
+// 3 x 3 neighbourhoods to the next list. list (kFineLists * track.list_cap
+// entries) and count (kFineLists counters) are valid device pointers either
+// way: the kernel reads both before it knows which it uses.
 hipError_t launch_fine_tiles(const FineGeo& g, const uint32_t* src, uint32_t* dst, int gens, const int* list,
-                             const int* count, const FineTrack& track, hipStream_t stream);
+                             const int* count, bool listed, const FineTrack& track, hipStream_t stream);
 
 // Batched grids: one workgroup per work item, all items of one tile shape.
 hipError_t launch_items(const TileShape& shape, const BatchGrid* grids, const WorkItem* items, int n_items,

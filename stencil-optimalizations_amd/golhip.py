@@ -99,7 +99,7 @@ class GolInfo(ctypes.Structure):
         ("tracked_blocks", ctypes.c_uint64),
         ("device_bytes", ctypes.c_uint64),
         ("native_layout", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("workgroups", ctypes.c_uint32),
     ]
 
 

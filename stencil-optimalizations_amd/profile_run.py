@@ -57,17 +57,29 @@ def main(what: str) -> None:
                 print(f"slab: 10000 generations, {ms:.2f} ms GPU", flush=True)
             info = g.info()
         print(f"slab: {info.launches} launches", flush=True)
-    elif what in ("active", "spacefiller"):
-        if what == "active":
+    elif what in ("active", "spacefiller", "active_tiles", "spacefiller_tiles"):
+        if what.startswith("active"):
             w = golhip.fill_patches(16384, 16384, 42, 256, 64, 0.35)
         else:
             w = golhip.fill_lexicon(16384, 16384, "spacefiller[8192, 8192]")
         with golhip.Grid(16384, 16384, kernel=golhip.KERNEL_ACTIVE, device=0) as g:
             g.upload(w)
-            ms = g.step_timed(20000)
-            info = g.info()
-        print(f"{what}: 20000 generations, {info.launches} launches, {ms:.2f} ms GPU, "
-              f"{info.active_tiles_total} active tile-blocks of {info.tiles * info.tracked_blocks}", flush=True)
+            if what.endswith("_tiles"):
+                # the tiles each block advanced: one block per call, read from
+                # the handle's running total (a CSV line per block)
+                print("block,tiles", flush=True)
+                before = 0
+                for b in range(20000 // g.info().temporal_steps):
+                    g.step(g.info().temporal_steps)
+                    total = g.info().active_tiles_total
+                    print(f"{b},{total - before}")
+                    before = total
+            else:
+                ms = g.step_timed(20000)
+                info = g.info()
+                print(f"{what}: 20000 generations, {info.launches} launches, {ms:.2f} ms GPU, "
+                      f"{info.active_tiles_total} active tile-blocks of {info.tiles * info.tracked_blocks}",
+                      flush=True)
     elif what in ("batch", "batch_queue"):
         specs = workloads.batched_uneven(4096)
         inputs = workloads.make_inputs(specs)
@@ -105,7 +117,9 @@ def main(what: str) -> None:
         print("copy: 10 x 256 MiB device copies", flush=True)
     else:
         raise SystemExit(f"unknown workload {what}")
-    print(f"done in {time.perf_counter() - t0:.1f} s", flush=True)
+    # the per-block tile modes print CSV on stdout
+    print(f"done in {time.perf_counter() - t0:.1f} s", flush=True,
+          file=sys.stderr if what.endswith("_tiles") else sys.stdout)
 
 
 if __name__ == "__main__":
