@@ -393,7 +393,7 @@ __device__ __forceinline__ void fine_load(const uint32_t* __restrict__ src, cons
 // Advances one tile, whose words fine_load read, by gens generations. With
 // tracking (tr.list_next set), a tile whose cells changed appends the tiles of
 // its 3 x 3 neighbourhood to the next block's list: a tile joins once, when
-// its mark goes from 0 to 1, to sub-list fine_sub_list(nx, ny) of the new
+// its mark goes from 0 to 1, to sub-list fine_sub_list(nx, ny, DW) of the new
 // tile, whose counter reserves its entry. Sub-lists spread the reservations,
 // device-scope atomics that one counter would take one after another, over
 // kFineLists counters in lines of their own. The wavefront clears the mark
@@ -469,7 +469,7 @@ __device__ __forceinline__ void fine_tile(const FineRow<DW>& r, uint32_t* __rest
         if (nx >= 0 && nx < g.tiles_x && ny >= 0 && ny < g.tiles_y) {
             const int n = ny * g.tiles_x + nx;
             if (atomicOr(&tr.mark_next[n], 1u) == 0u) {
-                const int j = fine_sub_list(nx, ny);
+                const int j = fine_sub_list(nx, ny, DW);
                 const int at = atomicAdd(tr.count_next + j * kFineCountStride, 1);
                 tr.list_next[static_cast<size_t>(j) * tr.list_cap + at] = n;
             }
@@ -477,11 +477,14 @@ __device__ __forceinline__ void fine_tile(const FineRow<DW>& r, uint32_t* __rest
     }
 }
 
-// A tracked block advances the tiles of kFineLists sub-lists: wavefront w
-// (of W = kFineWaves * gridDim.x, at least kFineLists) takes sub-list
-// w % kFineLists, entries r, r + step, ... with r = w / kFineLists and step
-// the number of wavefronts on that sub-list. An untracked block advances
-// tiles w, w + W, ... < dense_tiles. Either way a wavefront reads one tile
+// A tracked block advances the tiles of kFineLists sub-lists: workgroup b (of
+// at least kFineLists) takes sub-list b % kFineLists, the one its XCD owns,
+// and its wavefront v entries r, r + step, ... with r = kFineWaves *
+// (b / kFineLists) + v and step the number of wavefronts on that sub-list. An
+// untracked block advances tiles w, w + W, ... < dense_tiles, wavefront
+// numbers w going to the workgroups in runs of 128 per XCD (neighbouring
+// tiles, which share lines, on one L2), W = kFineWaves * gridDim.x. Either
+// way a wavefront reads one tile
 // ahead: the loads of the next tile are in flight while the current tile
 // advances (a block lists more tiles than the launch has wavefronts when much
 // of the grid is active). The entry after it is read before the tile
@@ -495,20 +498,20 @@ __global__ __launch_bounds__(64 * kFineWaves) void fine_tile_kernel(const uint32
                                                                     int dense_tiles, FineTrack tr) {
     if (tr.count_clear && blockIdx.x == 0 && threadIdx.x < kFineLists)
         tr.count_clear[threadIdx.x * kFineCountStride] = 0;
-    const int waves = static_cast<int>(gridDim.x) * kFineWaves;
-    // Workgroup b runs on XCD b % 8; runs of 128 consecutive wavefront
-    // numbers go to 32 workgroups of one XCD (one L2), the runs dealt to the
+    const int nwg = static_cast<int>(gridDim.x);
+    const int b = static_cast<int>(blockIdx.x);
+    const int v = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    // Untracked: workgroup b runs on XCD b % 8; runs of 128 consecutive
+    // wavefront numbers go to 32 workgroups of one XCD, the runs dealt to the
     // XCDs in turn. A bijection onto the workgroups when their number is a
     // multiple of 256; else workgroup b takes numbers 4b .. 4b + 3.
-    const int b = static_cast<int>(blockIdx.x);
-    const int group = (gridDim.x & 255) == 0 ? ((b >> 8) * 8 + (b & 7)) * 32 + ((b >> 3) & 31) : b;
-    const int w = group * kFineWaves + __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    const int group = (nwg & 255) == 0 ? ((b >> 8) * 8 + (b & 7)) * 32 + ((b >> 3) & 31) : b;
     // sub-list, first entry and step of a tracked block; for an untracked one
     // the "sub-list" is the whole range of tiles
-    const int sub = w % kFineLists;
+    const int sub = b % kFineLists;
     const int cap = listed ? tr.list_cap : dense_tiles;
-    const int step = listed ? (waves - sub + kFineLists - 1) / kFineLists : waves;
-    int at = listed ? w / kFineLists : w;
+    const int step = listed ? kFineWaves * ((nwg - sub + kFineLists - 1) / kFineLists) : nwg * kFineWaves;
+    int at = listed ? (b / kFineLists) * kFineWaves + v : group * kFineWaves + v;
     const int* entries = list + static_cast<size_t>(sub) * tr.list_cap;
     auto tile_at = [&](int i) { return listed ? __builtin_amdgcn_readfirstlane(entries[i]) : i; };
     // The first entry is read together with the sub-list's length, both loads

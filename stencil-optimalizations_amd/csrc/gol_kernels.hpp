@@ -122,20 +122,29 @@ inline int fine_workgroups(const FineGeo& g) { return g.workgroups; }
 // the current device (0 if unknown).
 int fine_resident_workgroups(int tile_dwords, int halo_rows, int64_t x_dw);
 
-// A block's list of tiles is kFineLists sub-lists: tile (tx, ty) goes to
-// sub-list (tx + 5 ty) mod kFineLists, so that a strip of tiles along a row,
-// a column or a diagonal spreads over all of them. Each row of tiles puts at
-// most ceil(tiles_x / kFineLists) tiles into one sub-list, so a sub-list
-// holds at most list_cap = tiles_y * ceil(tiles_x / kFineLists) entries. The
-// length of sub-list j is counts[j * kFineCountStride] (each counter in a
-// 128-byte line of its own).
+// A block's list of tiles is kFineLists sub-lists, one per XCD: sub-list j
+// is taken by the workgroups b with b % kFineLists == j, which run on XCD j.
+// Tile (tx, ty) goes to sub-list (c + 5 ty) mod kFineLists, c = the tile's
+// 128-byte line column (tx * tile_dwords / 32): the tiles that share the
+// lines of a row stay on one XCD, which fetches those lines once, and a strip
+// of activity along a row, a column or a diagonal spreads over all the
+// sub-lists. In each row of tiles a sub-list takes every kFineLists-th line
+// column, so it holds at most list_cap = tiles_y * (32 / tile_dwords) *
+// ceil(line columns / kFineLists) entries. The length of sub-list j is
+// counts[j * kFineCountStride] (each counter in a 128-byte line of its own).
 constexpr int kFineLists = 8;
 constexpr int kFineCountStride = 32;
 constexpr int kFineCounts = kFineLists * kFineCountStride;  // ints of one list's counters
-__host__ __device__ inline int fine_sub_list(int tx, int ty) { return (tx + 5 * ty) & (kFineLists - 1); }
-inline int fine_list_cap(const FineGeo& g) { return g.tiles_y * ((g.tiles_x + kFineLists - 1) / kFineLists); }
-// Workgroups a tracked launch needs at least: one wavefront per sub-list.
-constexpr int kFineMinWorkgroups = kFineLists / kFineWaves;
+__host__ __device__ inline int fine_sub_list(int tx, int ty, int tile_dwords) {
+    return (((tx * tile_dwords) >> 5) + 5 * ty) & (kFineLists - 1);
+}
+inline int fine_list_cap(const FineGeo& g) {
+    const int per_line = 32 / g.tile_dwords;
+    const int columns = (g.tiles_x + per_line - 1) / per_line;
+    return g.tiles_y * per_line * ((columns + kFineLists - 1) / kFineLists);
+}
+// Workgroups a tracked launch needs at least: one per sub-list.
+constexpr int kFineMinWorkgroups = kFineLists;
 
 // Change-tracking state of one block (all null: no tracking).
 struct FineTrack {
