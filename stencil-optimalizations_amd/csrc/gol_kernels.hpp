@@ -98,7 +98,7 @@ int tile_occupancy(const TileShape& shape);
 // the tile's rows and halo_rows (= T) halo rows above and below:
 // tile_rows = 64 - 2T.
 constexpr int kFineWaves = 4;      // wavefronts (tiles) per workgroup
-constexpr int kFineDefaultDW = 4;  // tile width in dwords when gol_config.lane_dwords is 0
+constexpr int kFineDefaultDW = 2;  // tile width in dwords when gol_config.lane_dwords is 0
 constexpr int kFineMaxSteps = 21;  // T <= tile_rows: a tile is at least as tall as T generations reach
 // Blocks of T <= 16 generations need at most 16 halo cells on each side of a
 // tile: one dword holds both (the ring form, DW + 1 dwords per lane); deeper

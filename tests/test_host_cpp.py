@@ -78,7 +78,7 @@ def test_cli_lexicon_and_time_budget():
 def test_cli_touched_tile_stats():
     """--collect-touched-tiles-stats with change tracking: the reference's three
     lines (cuda_local_one_cell.hpp:130-155), one entry per generation. A glider
-    on an empty 1024 x 768 grid: 8 x 24 tiles of 128 x 32 cells, blocks of 16
+    on an empty 1024 x 768 grid: 16 x 24 tiles of 64 x 32 cells, blocks of 16
     generations; the first block and the trailing 4-generation block advance
     every tile, the rest only the glider's neighbourhood."""
     print("[case] stencils touched-tile stats", flush=True)
@@ -95,7 +95,7 @@ def test_cli_touched_tile_stats():
                 lines[key] = [int(v) for v in line[len(key):].strip().strip(";").split(";")]
     changed, copied, untouched = lines["Changed_tiles:"], lines["Copied_tiles:"], lines["Untouched_tiles:"]
     assert len(changed) == len(copied) == len(untouched) == 100
-    tiles = 8 * 24
+    tiles = 16 * 24
     assert all(c + u == tiles for c, u in zip(changed, untouched)) and not any(copied)
     assert changed[:16] == [tiles] * 16 and changed[96:] == [tiles] * 4
     assert all(1 <= c <= 16 for c in changed[16:96]), changed
