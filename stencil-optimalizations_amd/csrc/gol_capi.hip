@@ -374,7 +374,9 @@ int enqueue_persistent(gol_handle* h, uint64_t generations) {
         hipError_t e = launch_slab(h->shape, a, h->stream);
         if (e != hipSuccess) return fail_hip(e, "slab_kernel launch");
         const uint64_t total = blocks + (last ? 1 : 0);
-        h->slab_epoch += total > 0 ? total - 1 : 0;  // exchanges happen between blocks
+        // exchanges happen between blocks, each counted by every exporting wavefront
+        h->slab_epoch += (total > 0 ? total - 1 : 0) *
+                         static_cast<uint64_t>(slab_publishers(h->shape.band_rows, h->shape.waves, h->steps));
         h->cur ^= 1;
         h->launches += 1;
         const uint64_t done = blocks * T + last;

@@ -836,6 +836,28 @@ __device__ __forceinline__ unsigned long long fetch_pair(const unsigned long lon
 
 // The block loop of one slab; the row and column masks are fixed for the
 // slab, so each instantiation holds one copy of the band update.
+//
+// Between blocks the exchange runs beside the next block's first generation:
+//   1. after a block, each wavefront holding exported rows stores them
+//      (device scope) and goes on;
+//   2. each exporting wavefront waits for its stores' acknowledgement
+//      (s_waitcnt 0) and then adds one to the slab's counter (device-scope
+//      atomic);
+//   3. each wavefront holding halo rows polls its neighbour's counter (lane 0)
+//      until all the neighbour's exporting wavefronts have counted the block,
+//      then loads the rows into its registers.
+// Where every band boundary lies among the own rows (band_rows > T, the
+// 80-row slab at T <= 8), no band reads a halo row across a boundary, so the
+// next block's first generation starts with its LDS edge exchange and barrier
+// at once: after that barrier the exporting wavefronts count and the
+// importing ones import, and then each advances its band, while the others
+// advance theirs at once. Otherwise every exporting wavefront counts, and
+// every importing one imports, before that barrier. No wait depends on a
+// barrier the neighbour has not passed (a slab counts before it waits, and
+// the barrier before the count needs no import), so the chain of waits
+// between slabs always ends. Waits are bounded; a
+// wavefront whose wait runs out clears keep_going, and every wavefront leaves
+// after the next barrier.
 template <int DW, int BR, int NW, bool MASK_ROWS, bool MASK_COLS>
 __device__ __forceinline__ bool slab_blocks(const SlabArgs& a, uint32_t (&s)[BR][DW], EdgeRows<DW, NW>& lds,
                                             int* keep_going, int lane, int w, int row_lo, int row_hi,
@@ -849,12 +871,106 @@ __device__ __forceinline__ bool slab_blocks(const SlabArgs& a, uint32_t (&s)[BR]
     // wavefronts holding halo rows import them (wave-uniform)
     const bool import_top = ty > 0 && w * BR < T;
     const bool import_bottom = ty + 1 < a.tiles_y && (w + 1) * BR > tile_rows - T;
+    const bool imports = import_top || import_bottom;
+    const bool exports = (w * BR < 2 * T && (w + 1) * BR > T) ||
+                         (w * BR < tile_rows - T && (w + 1) * BR > tile_rows - 2 * T);
+    const unsigned long long per_block = static_cast<unsigned long long>(slab_publishers(BR, NW, T));
     const int spins = a.spin_limit > 0 ? a.spin_limit : kSlabSpinLimit;
+    auto count_publication = [&]() {
+        __builtin_amdgcn_s_waitcnt(0);  // this wavefront's exported rows are acknowledged
+        if (lane == 0) __hip_atomic_fetch_add(&a.flags[ty], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    // Subscribe and import: lane 0 polls the neighbour the rows come from
+    // until its exporting wavefronts have all counted block b - 1; the
+    // wavefront then loads the rows straight into its registers (device-scope
+    // loads issued after the counter load returned, so they see the rows the
+    // neighbour's acknowledged stores wrote). Lane i's dwords 2k, 2k+1 are
+    // pair slot k*64 + i of a row.
+    auto import_rows = [&](int par_b, int b) {
+        const unsigned long long wanted = a.epoch + static_cast<unsigned long long>(b) * per_block + a.wait_ahead;
+        int ok = 1;
+        if (lane == 0) {
+            if (import_top) ok = wait_counter(&a.flags[ty - 1], wanted, a.error, spins) ? 1 : 0;
+            if (ok && import_bottom) ok = wait_counter(&a.flags[ty + 1], wanted, a.error, spins) ? 1 : 0;
+        }
+        ok = __builtin_amdgcn_readfirstlane(ok);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if (ok) {
+            const unsigned long long* up =
+                a.xbuf + (static_cast<size_t>(par_b) * a.tiles_y + (ty > 0 ? ty - 1 : 0)) * 2 * plane + plane;
+            const unsigned long long* dn =
+                a.xbuf + (static_cast<size_t>(par_b) * a.tiles_y + (ty + 1 < a.tiles_y ? ty + 1 : ty)) * 2 * plane;
+#pragma unroll
+            for (int r = 0; r < BR; ++r) {
+                const int tr = w * BR + r;
+                const bool top = import_top && tr < T;
+                const bool bottom = import_bottom && tr >= tile_rows - T;
+                if (top || bottom) {
+                    const unsigned long long* row =
+                        top ? up + static_cast<size_t>(tr) * RP : dn + static_cast<size_t>(tr - (tile_rows - T)) * RP;
+#pragma unroll
+                    for (int k = 0; k < DW / 2; ++k) {
+                        const unsigned long long v = fetch_pair(row + k * 64 + lane);
+                        s[r][2 * k] = static_cast<uint32_t>(v) & colmask[2 * k];
+                        s[r][2 * k + 1] = static_cast<uint32_t>(v >> 32) & colmask[2 * k + 1];
+                    }
+                }
+            }
+        } else if (lane == 0) {
+            *keep_going = 0;
+        }
+    };
+    // Whether the import may follow the first generation's LDS edge exchange:
+    // only if no band edge row that another band reads is a halo row, i.e.
+    // every band boundary lies among the own rows (band_rows > T). Otherwise
+    // every exporting wavefront counts and every importing one imports before
+    // that exchange.
+    const bool late = BR > T;
+    // The wavefronts on the exchange's path (exporting or importing) issue
+    // ahead of the interior ones sharing their SIMD, so they finish each
+    // generation first: the exported rows are stored, and acknowledged,
+    // earlier, and the imported rows are advanced as soon as they arrive.
+    if (exports || imports) __builtin_amdgcn_s_setprio(1);
     for (int b = 0; b < total_blocks; ++b) {
         const int gens = b < a.blocks ? T : a.last_gens;
-        run_generations<DW, BR, NW, MASK_ROWS, MASK_COLS>(s, lds, gens, lane, w, row_lo, row_hi, colmask);
+        const int par_b = (b - 1) & 1;  // exchange parity of the previous block's rows
+        // The block's first generation, with the import (its own copy of the
+        // band update: a branch around the import inside the generation loop
+        // would make every generation reshuffle the band's registers).
+        {
+            if (b > 0 && !late) {
+                if (exports) count_publication();
+                if (imports) import_rows(par_b, b);
+            }
+            lds_store_seg<DW>(lds.tops[0][w], lane, s[0]);
+            lds_store_seg<DW>(lds.bottoms[0][w + 1], lane, s[BR - 1]);
+            __syncthreads();
+            if (!*keep_going) return false;  // a wait ran out: the whole workgroup leaves; the error is recorded
+            uint32_t above[DW], below[DW];
+            lds_load_seg<DW>(lds.bottoms[0][w], lane, above);
+            lds_load_seg<DW>(lds.tops[0][w + 1], lane, below);
+            if (b > 0 && late) {
+                if (exports) count_publication();
+                if (imports) import_rows(par_b, b);
+            }
+            advance_band<DW, BR, MASK_ROWS, MASK_COLS>(s, above, below, row_lo, row_hi, colmask);
+        }
+        // the block's other generations
+        for (int g = 1; g < gens; ++g) {
+            const int par = g & 1;
+            lds_store_seg<DW>(lds.tops[par][w], lane, s[0]);
+            lds_store_seg<DW>(lds.bottoms[par][w + 1], lane, s[BR - 1]);
+            __syncthreads();
+            if (!*keep_going) return false;
+            uint32_t above[DW], below[DW];
+            lds_load_seg<DW>(lds.bottoms[par][w], lane, above);
+            lds_load_seg<DW>(lds.tops[par][w + 1], lane, below);
+            advance_band<DW, BR, MASK_ROWS, MASK_COLS>(s, above, below, row_lo, row_hi, colmask);
+        }
         if (b + 1 == total_blocks) break;
 
+        // publish this block's exported rows (device-scope stores; counted in
+        // the next block's first generation)
         const int par = b & 1;
         unsigned long long* mine = a.xbuf + (static_cast<size_t>(par) * a.tiles_y + ty) * 2 * plane;
 #pragma unroll
@@ -865,60 +981,9 @@ __device__ __forceinline__ bool slab_blocks(const SlabArgs& a, uint32_t (&s)[BR]
                 publish_seg<DW>(mine + plane + static_cast<size_t>(tr - (tile_rows - 2 * T)) * RP, lane, s[r]);
             __builtin_amdgcn_sched_barrier(0);
         }
-        // Publish: every wave waits for its device-scope stores to be
-        // acknowledged (vmcnt 0) before the barrier; then one thread bumps the
-        // counter.
-        if (threadIdx.x == 0) *keep_going = 1;
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        const unsigned long long target = a.epoch + static_cast<unsigned long long>(b) + 1;
-        if (threadIdx.x == 0) __hip_atomic_store(&a.flags[ty], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-
-        // Subscribe and import: lane 0 of each wavefront holding halo rows polls
-        // the neighbour they come from; once its counter is seen the wavefront
-        // loads the neighbour's rows straight into its registers (device-scope
-        // loads issued after the counter load returned, so they see the rows
-        // the neighbour's acknowledged stores wrote). Lane i's dwords 2k, 2k+1
-        // are pair slot k*64 + i of a row.
-        const unsigned long long wanted = target + a.wait_ahead;
-        int ok = 1;
-        if (import_top || import_bottom) {
-            if (lane == 0) {
-                if (import_top) ok = wait_counter(&a.flags[ty - 1], wanted, a.error, spins) ? 1 : 0;
-                if (ok && import_bottom) ok = wait_counter(&a.flags[ty + 1], wanted, a.error, spins) ? 1 : 0;
-            }
-            ok = __builtin_amdgcn_readfirstlane(ok);
-            __atomic_signal_fence(__ATOMIC_SEQ_CST);
-            if (ok) {
-                const unsigned long long* above =
-                    a.xbuf + (static_cast<size_t>(par) * a.tiles_y + (ty > 0 ? ty - 1 : 0)) * 2 * plane + plane;
-                const unsigned long long* below =
-                    a.xbuf + (static_cast<size_t>(par) * a.tiles_y + (ty + 1 < a.tiles_y ? ty + 1 : ty)) * 2 * plane;
-#pragma unroll
-                for (int r = 0; r < BR; ++r) {
-                    const int tr = w * BR + r;
-                    const bool top = import_top && tr < T;
-                    const bool bottom = import_bottom && tr >= tile_rows - T;
-                    if (top || bottom) {
-                        const unsigned long long* row =
-                            top ? above + static_cast<size_t>(tr) * RP
-                                : below + static_cast<size_t>(tr - (tile_rows - T)) * RP;
-#pragma unroll
-                        for (int k = 0; k < DW / 2; ++k) {
-                            const unsigned long long v = fetch_pair(row + k * 64 + lane);
-                            s[r][2 * k] = static_cast<uint32_t>(v) & colmask[2 * k];
-                            s[r][2 * k + 1] = static_cast<uint32_t>(v >> 32) & colmask[2 * k + 1];
-                        }
-                    }
-                }
-            } else if (lane == 0) {
-                *keep_going = 0;
-            }
-        }
-        __syncthreads();
-        if (!*keep_going) return false;  // the whole workgroup leaves together; the error is recorded
     }
-    return true;
+    __syncthreads();
+    return *keep_going != 0;
 }
 
 template <int DW, int BR, int NW, bool MASK_COLS>
@@ -939,6 +1004,7 @@ __global__ __launch_bounds__(64 * NW) void slab_kernel(SlabArgs a) {
 #pragma unroll
     for (int r = 0; r < BR; ++r) load_seg<DW>(a.src, g.x_dw, g.y_rows, gy0 + r, gx0, vec_ok, s[r]);
     clear_edge_slots<DW, NW>(lds, lane, w);
+    if (threadIdx.x == 0) keep_going = 1;
 
     uint32_t colmask[DW];
 #pragma unroll

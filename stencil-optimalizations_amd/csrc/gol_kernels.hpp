@@ -203,12 +203,26 @@ struct SlabArgs {
     int last_gens;              // generations of a final shorter block (0: none)
     int tiles_y;                // workgroups
     unsigned long long* xbuf;   // [2][tiles_y][2][T][32 * DW] exchanged halo rows, lane-interleaved pairs
-    unsigned long long* flags;  // [tiles_y] blocks published so far, offset by epoch
-    unsigned long long epoch;   // flag value every slab holds before this launch
+    unsigned long long* flags;  // [tiles_y] publications counted so far (slab_publishers per block), offset by epoch
+    unsigned long long epoch;   // counter value every slab holds before this launch
     unsigned int* error;        // raised when a neighbour wait runs out
     int spin_limit;             // polls before a wait gives up; 0: the built-in limit
     unsigned long long wait_ahead;  // 0; testing the failure path: waits for counts no slab publishes
 };
+
+// Wavefronts of a slab holding exported rows ([T, 2T) or [R - 2T, R - T) of
+// its R = band_rows * waves rows): each counts its own publication, so a
+// slab's block counter grows by this many per exchanged block. The same for
+// every slab (every slab publishes both sides).
+__host__ __device__ inline int slab_publishers(int band_rows, int waves, int steps) {
+    const int R = band_rows * waves, T = steps;
+    int n = 0;
+    for (int w = 0; w < waves; ++w) {
+        const int b0 = w * band_rows, b1 = b0 + band_rows;
+        if ((b0 < 2 * T && b1 > T) || (b0 < R - T && b1 > R - 2 * T)) ++n;
+    }
+    return n;
+}
 
 // Workgroups of the slab kernel that can be resident at once (false if the
 // device cannot launch cooperatively); mask_cols selects the variant for rows

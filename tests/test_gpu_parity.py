@@ -196,6 +196,27 @@ def test_persistent_slabs_against_oracle(x, y, gens, steps):
     assert persistent in (0, 1)
 
 
+@pytest.mark.parametrize("band_rows,waves", [(10, 8), (8, 8), (5, 16), (4, 16)])
+@pytest.mark.parametrize("steps", [2, 5, 8])
+def test_persistent_slab_shapes_against_oracle(band_rows, waves, steps):
+    """Every compiled slab shape at several block depths, on a full-width grid
+    with a ragged last slab and a partial last block. On the 80-row slab at
+    these depths every band boundary lies among the own rows, so the halo
+    import follows the next block's first barrier; on the other shapes halo
+    rows sit on band edges and the import comes first."""
+    x, y = 16384, 1000
+    gens = 4 * steps + 3
+    note("persistent shape", band_rows, waves, steps, gens)
+    w = ol.pack(ol.bernoulli_cells(x, y, band_rows * 100 + steps, 0.4))
+    ref = ol.bitpar_run(w, x, y, gens)
+    with golhip.Grid(x, y, kernel=golhip.KERNEL_TEMPORAL, temporal_steps=steps, lane_dwords=8,
+                     band_rows=band_rows, waves=waves) as g:
+        assert g.info().persistent == 1
+        g.upload(w)
+        g.step(gens)
+        assert np.array_equal(g.download(), ref)
+
+
 def test_persistent_repeated_launches_match_fixture():
     """Ten persistent launches of 10000 generations on one handle, the input
     coming alternately from a device buffer and from the host (the benchmark's
