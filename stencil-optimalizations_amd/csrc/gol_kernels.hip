@@ -908,11 +908,14 @@ __device__ __forceinline__ bool slab_blocks(const SlabArgs& a, uint32_t (&s)[BR]
                 if (top || bottom) {
                     const unsigned long long* row =
                         top ? up + static_cast<size_t>(tr) * RP : dn + static_cast<size_t>(tr - (tile_rows - T)) * RP;
+                    // the neighbour's rows are already clear outside the grid
+                    // (its band update masks them), so each pair loads
+                    // straight into the band's registers
 #pragma unroll
                     for (int k = 0; k < DW / 2; ++k) {
                         const unsigned long long v = fetch_pair(row + k * 64 + lane);
-                        s[r][2 * k] = static_cast<uint32_t>(v) & colmask[2 * k];
-                        s[r][2 * k + 1] = static_cast<uint32_t>(v >> 32) & colmask[2 * k + 1];
+                        s[r][2 * k] = static_cast<uint32_t>(v);
+                        s[r][2 * k + 1] = static_cast<uint32_t>(v >> 32);
                     }
                 }
             }
