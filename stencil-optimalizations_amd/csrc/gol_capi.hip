@@ -65,9 +65,12 @@ TileShape default_shape(int64_t x_dw, int64_t y_rows) {
 //            it except for about one tile's share;
 //   launch:  about 4 us between dependent launches;
 //   persistent slabs: one tile per CU for the whole run, no per-block load,
-//            store or launch, about 6.5 us of halo exchange per block (with
-//            the compute term, the measured 35.3 us per block of 8
-//            generations on 16384^2).
+//            store or launch; the halo exchange runs beside the next block's
+//            first generation and leaves about 0.6 + 0.2 * T us per block
+//            exposed (with the compute term, the measured 14.5 us per block of
+//            4 generations on 72-row slabs and 7.5 us per block of 2 on
+//            16384^2; 80-row slabs at 8 generations measure 31.9 us, 0.9 us
+//            above the model).
 double model_us_per_generation(const TileShape& s, const TileGeometry& g, int steps, int occupancy,
                                bool persistent) {
     const double cus = 256.0;
@@ -76,7 +79,7 @@ double model_us_per_generation(const TileShape& s, const TileGeometry& g, int st
     const double dwords_per_tile = static_cast<double>(s.band_rows) * s.waves * 64.0 * s.lane_dwords;
     const double ops = 12.7 + 8.0 / s.band_rows;
     const double compute_us = waves_of_tiles * dwords_per_tile * steps * ops / 64.0 / 2400.0;
-    if (persistent) return (compute_us + 6.5) / steps;
+    if (persistent) return (compute_us + 0.6 + 0.2 * steps) / steps;
     const double bytes = tiles * (static_cast<double>(g.tile_rows) + g.stride_rows) * 64.0 * s.lane_dwords * 4.0;
     const double memory_us = bytes / 5.0e6;
     const double busy = occupancy >= 2 && waves_of_tiles >= 2

@@ -1032,7 +1032,7 @@ __global__ __launch_bounds__(64 * NW) void slab_kernel(SlabArgs a) {
 }  // namespace
 
 const TileShape kTileShapes[] = {
-    {8, 8, 8}, {8, 10, 8}, {8, 5, 16}, {8, 4, 16}, {4, 8, 8}, {4, 8, 16}, {2, 16, 8}, {1, 8, 8}, {1, 16, 8},
+    {8, 8, 8}, {8, 10, 8}, {8, 9, 8}, {8, 5, 16}, {8, 4, 16}, {4, 8, 8}, {4, 8, 16}, {2, 16, 8}, {1, 8, 8}, {1, 16, 8},
     {1, 24, 8}, {1, 32, 8}, {1, 32, 16},
 };
 
@@ -1089,6 +1089,7 @@ bool plan_tiles(const TileShape& shape, int64_t x_dw, int64_t y_rows, int max_ge
 #define GOLHIP_FOR_EACH_SHAPE(X) \
     X(8, 8, 8)                     \
     X(8, 10, 8)                    \
+    X(8, 9, 8)                     \
     X(8, 5, 16)                    \
     X(8, 4, 16)                    \
     X(4, 8, 8)                     \
@@ -1275,8 +1276,11 @@ hipError_t launch_zero(unsigned int* words, int n, hipStream_t stream) {
 }
 
 // The slab kernels compiled: full-width lanes of 8 dwords (rows of up to 16384 cells).
+// 72-row slabs (9-row bands) at 4 generations per block keep 64 own rows each,
+// as 80-row slabs at 8 do, with half the halo rows to recompute.
 #define GOLHIP_FOR_EACH_SLAB_SHAPE(X) \
     X(8, 10, 8)                        \
+    X(8, 9, 8)                         \
     X(8, 5, 16)                        \
     X(8, 4, 16)                        \
     X(8, 8, 8)

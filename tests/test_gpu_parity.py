@@ -196,14 +196,15 @@ def test_persistent_slabs_against_oracle(x, y, gens, steps):
     assert persistent in (0, 1)
 
 
-@pytest.mark.parametrize("band_rows,waves", [(10, 8), (8, 8), (5, 16), (4, 16)])
-@pytest.mark.parametrize("steps", [2, 5, 8])
+@pytest.mark.parametrize("band_rows,waves", [(10, 8), (9, 8), (8, 8), (5, 16), (4, 16)])
+@pytest.mark.parametrize("steps", [2, 4, 5, 8])
 def test_persistent_slab_shapes_against_oracle(band_rows, waves, steps):
     """Every compiled slab shape at several block depths, on a full-width grid
-    with a ragged last slab and a partial last block. On the 80-row slab at
-    these depths every band boundary lies among the own rows, so the halo
-    import follows the next block's first barrier; on the other shapes halo
-    rows sit on band edges and the import comes first."""
+    with a ragged last slab and a partial last block. Where the bands are
+    taller than the depth (the 80- and 72-row slabs) every band boundary lies
+    among the own rows, so the halo import follows the next block's first
+    barrier; elsewhere halo rows sit on band edges and the import comes
+    first."""
     x, y = 16384, 1000
     gens = 4 * steps + 3
     note("persistent shape", band_rows, waves, steps, gens)
