@@ -456,8 +456,8 @@ def main() -> int:
     ap.add_argument("--served-load", type=float, default=0.8)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the engine has no "
-              f"multi-GPU path in this build", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the engine runs on one GPU "
+              f"(sharding across GPUs is a design, not code)", file=sys.stderr)
         return 2
 
     kernel = golhip.KERNEL_TEMPORAL if args.kernel == "temporal" else golhip.KERNEL_PLAIN
