@@ -167,6 +167,8 @@ struct gol_handle {
     uint64_t generations = 0;
     // change tracking (GOL_KERNEL_ACTIVE): small tiles, one wavefront each
     FineGeo fine{};
+    uint32_t* fbuf[2] = {nullptr, nullptr};  // the grid tile-major (fine_word_index), fine_words(fine) dwords each
+    int fcur = 0;                 // fbuf[fcur] holds the state while a step runs
     uint32_t* marks = nullptr;    // [3][n_tiles] tile is on list k (lists rotate by block)
     int* lists = nullptr;         // [3][kFineLists][fine_list_cap] active-tile lists
     int* counts = nullptr;        // [3][kFineLists][kFineCountStride] sub-list lengths
@@ -283,7 +285,7 @@ int plan(gol_handle* h) {
         // wavefronts loop over list slots: launch no more than the device
         // holds at once
         const int all = (h->n_tiles + kFineWaves - 1) / kFineWaves;
-        const int resident = fine_resident_workgroups(dw, steps, h->x_dw);
+        const int resident = fine_resident_workgroups(dw, steps);
         h->fine.workgroups = std::max(resident > 0 ? std::min(all, resident) : all, kFineMinWorkgroups);
         h->shape = TileShape{dw, 1, kFineWaves};
         return GOL_OK;
@@ -418,14 +420,48 @@ int enqueue_active_block(gol_handle* h) {
     tr.count_next = h->counts + k1 * kFineCounts;
     tr.count_clear = h->counts + k2 * kFineCounts;
     tr.wg_active = h->wg_active;
-    e = launch_fine_tiles(h->fine, h->buf[h->cur], h->buf[h->cur ^ 1], h->steps, h->lists + k * list_len,
+    e = launch_fine_tiles(h->fine, h->fbuf[h->fcur], h->fbuf[h->fcur ^ 1], h->steps, h->lists + k * list_len,
                           h->counts + k * kFineCounts, h->flags_valid, tr, h->stream);
     if (e != hipSuccess) return fail_hip(e, "fine_tile_kernel launch");
     h->flags_valid = true;
-    h->cur ^= 1;
+    h->fcur ^= 1;
     h->launches += 1;
     h->blocks += 1;
     h->generations += static_cast<uint64_t>(h->steps);
+    return GOL_OK;
+}
+
+// Change tracking: the grid goes tile-major into fbuf[fcur] (the other
+// buffer keeps the tiles the last block did not write, which equal their
+// state then), blocks of `steps` generations run on the tile-major buffers,
+// and the result comes back row-major into buf[cur]. The buffers never hold
+// another state between steps: uploads reset the lists, so the first block
+// after one writes every tile.
+int enqueue_active(gol_handle* h, uint64_t generations) {
+    if (generations == 0) return GOL_OK;
+    hipError_t e = launch_fine_layout(h->fine, h->buf[h->cur], h->fbuf[h->fcur], true, h->stream);
+    if (e != hipSuccess) return fail_hip(e, "change-tracking layout launch");
+    h->launches += 1;
+    uint64_t left = generations;
+    while (left >= static_cast<uint64_t>(h->steps)) {
+        int rc = enqueue_active_block(h);
+        if (rc) return rc;
+        left -= static_cast<uint64_t>(h->steps);
+    }
+    if (left > 0) {
+        // a shorter block runs every tile: an oscillator that returns to its
+        // state after `steps` generations need not after fewer
+        h->flags_valid = false;
+        e = launch_fine_tiles(h->fine, h->fbuf[h->fcur], h->fbuf[h->fcur ^ 1], static_cast<int>(left), h->lists,
+                              h->counts, false, FineTrack{}, h->stream);
+        if (e != hipSuccess) return fail_hip(e, "fine_tile_kernel launch");
+        h->fcur ^= 1;
+        h->launches += 1;
+        h->generations += left;
+    }
+    e = launch_fine_layout(h->fine, h->fbuf[h->fcur], h->buf[h->cur], false, h->stream);
+    if (e != hipSuccess) return fail_hip(e, "change-tracking layout launch");
+    h->launches += 1;
     return GOL_OK;
 }
 
@@ -446,16 +482,7 @@ int enqueue(gol_handle* h, uint64_t generations) {
         }
         return GOL_OK;
     }
-    if (h->kernel == GOL_KERNEL_ACTIVE) {
-        while (left >= static_cast<uint64_t>(h->steps)) {
-            int rc = enqueue_active_block(h);
-            if (rc) return rc;
-            left -= static_cast<uint64_t>(h->steps);
-        }
-        // a shorter block runs every tile: an oscillator that returns to its
-        // state after `steps` generations need not after fewer
-        if (left > 0) h->flags_valid = false;
-    }
+    if (h->kernel == GOL_KERNEL_ACTIVE) return enqueue_active(h, generations);
     while (left > 0) {
         uint32_t* src = h->buf[h->cur];
         uint32_t* dst = h->buf[h->cur ^ 1];
@@ -464,10 +491,6 @@ int enqueue(gol_handle* h, uint64_t generations) {
         if (h->kernel == GOL_KERNEL_PLAIN) {
             e = launch_plain(static_cast<int>(h->plain_dw), src, dst, h->x_dw, h->y_rows, h->stream);
             done = 1;
-        } else if (h->kernel == GOL_KERNEL_ACTIVE) {
-            done = std::min<uint64_t>(left, static_cast<uint64_t>(h->steps));
-            e = launch_fine_tiles(h->fine, src, dst, static_cast<int>(done), h->lists, h->counts, false, FineTrack{},
-                                  h->stream);
         } else {
             done = std::min<uint64_t>(left, static_cast<uint64_t>(h->steps));
             e = launch_tiles(h->geom, src, dst, h->x_dw, h->y_rows, static_cast<int>(done), h->stream);
@@ -543,7 +566,11 @@ int gol_create(const gol_config* config, gol_handle** out) {
             (e = hipMalloc(&h->lists, 3 * list_len * sizeof(int))) != hipSuccess ||
             (e = hipMalloc(&h->counts, 3 * kFineCounts * sizeof(int))) != hipSuccess ||
             (e = hipMalloc(&h->wg_active, nwg * sizeof(unsigned long long))) != hipSuccess ||
-            (e = hipMemsetAsync(h->wg_active, 0, nwg * sizeof(unsigned long long), h->stream)) != hipSuccess) {
+            (e = hipMemsetAsync(h->wg_active, 0, nwg * sizeof(unsigned long long), h->stream)) != hipSuccess ||
+            (e = hipMalloc(&h->fbuf[0], fine_words(h->fine) * sizeof(uint32_t))) != hipSuccess ||
+            (e = hipMalloc(&h->fbuf[1], fine_words(h->fine) * sizeof(uint32_t))) != hipSuccess ||
+            (e = hipMemsetAsync(h->fbuf[0], 0, fine_words(h->fine) * sizeof(uint32_t), h->stream)) != hipSuccess ||
+            (e = hipMemsetAsync(h->fbuf[1], 0, fine_words(h->fine) * sizeof(uint32_t), h->stream)) != hipSuccess) {
             rc = fail_hip(e, "gol_create (change tracking state)");
             gol_destroy(h);
             return rc;
@@ -735,6 +762,8 @@ void gol_destroy(gol_handle* h) {
         if (h->lists) (void)hipFree(h->lists);
         if (h->counts) (void)hipFree(h->counts);
         if (h->wg_active) (void)hipFree(h->wg_active);
+        if (h->fbuf[0]) (void)hipFree(h->fbuf[0]);
+        if (h->fbuf[1]) (void)hipFree(h->fbuf[1]);
         if (h->xbuf) (void)hipFree(h->xbuf);
         if (h->slab_flags) (void)hipFree(h->slab_flags);
         if (h->slab_error) (void)hipFree(h->slab_error);

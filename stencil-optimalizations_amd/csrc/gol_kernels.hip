@@ -312,82 +312,86 @@ __global__ __launch_bounds__(64 * NW) void tile_kernel(const uint32_t* __restric
 // barrier, and the wavefronts of a workgroup take independent tiles. A halo
 // cell is wrong after g generations only if it lies within g cells of the
 // wavefront's edge; after T generations the tile's own cells are exact.
+//
+// The kernel reads and writes the grid tile-major (fine_word_index): the
+// tiles in row-major order, each tile_rows x DW dwords, row-major inside. A
+// wavefront's own rows are then one contiguous run of 4 * DW * tile_rows
+// bytes, its halo rows the end of the tile above and the start of the tile
+// below, and its west and east halo dwords one dword of every row of the
+// neighbouring tiles' columns: 12 lines of 128 bytes for a 64 x 32 tile and
+// its halo, where the row-major grid puts each of the 64 rows in a line of
+// its own. The step converts the grid to this layout before its first block
+// and back after its last (fine_pack_kernel, fine_unpack_kernel).
 // ---------------------------------------------------------------------------
 template <int N, bool MASK, bool RING>
-__device__ __forceinline__ void fine_generations(uint32_t (&s)[N], int gens, uint32_t rowmask,
-                                                 const uint32_t (&colmask)[N]) {
-    for (int g = 0; g < gens; ++g) {
-        uint32_t h0[N], h1[N], e0[N], e1[N];
+__device__ __forceinline__ void fine_generation(uint32_t (&s)[N], uint32_t rowmask, const uint32_t (&colmask)[N]) {
+    uint32_t h0[N], h1[N], e0[N], e1[N];
 #pragma unroll
-        for (int j = 0; j < N; ++j) {
-            const uint32_t w = west_of(j > 0 ? s[j - 1] : (RING ? s[N - 1] : 0u), s[j]);
-            const uint32_t e = east_of(s[j], j + 1 < N ? s[j + 1] : (RING ? s[0] : 0u));
-            const RowSum r = row_sum(w, s[j], e);
-            h0[j] = r.h0;
-            h1[j] = r.h1;
-            e0[j] = r.e0;
-            e1[j] = r.e1;
-        }
+    for (int j = 0; j < N; ++j) {
+        const uint32_t w = west_of(j > 0 ? s[j - 1] : (RING ? s[N - 1] : 0u), s[j]);
+        const uint32_t e = east_of(s[j], j + 1 < N ? s[j + 1] : (RING ? s[0] : 0u));
+        const RowSum r = row_sum(w, s[j], e);
+        h0[j] = r.h0;
+        h1[j] = r.h1;
+        e0[j] = r.e0;
+        e1[j] = r.e1;
+    }
 #pragma unroll
-        for (int j = 0; j < N; ++j) {
-            const uint32_t a0 = from_lower_lane(h0[j]), a1 = from_lower_lane(h1[j]);  // row above
-            const uint32_t b0 = from_upper_lane(h0[j]), b1 = from_upper_lane(h1[j]);  // row below
-            uint32_t v = next_state(a0, a1, e0[j], e1[j], b0, b1, s[j]);
-            if (MASK) v &= rowmask & colmask[j];
-            s[j] = v;
-        }
+    for (int j = 0; j < N; ++j) {
+        const uint32_t a0 = from_lower_lane(h0[j]), a1 = from_lower_lane(h1[j]);  // row above
+        const uint32_t b0 = from_upper_lane(h0[j]), b1 = from_upper_lane(h1[j]);  // row below
+        uint32_t v = next_state(a0, a1, e0[j], e1[j], b0, b1, s[j]);
+        if (MASK) v &= rowmask & colmask[j];
+        s[j] = v;
     }
 }
 
-// The words a lane reads for one tile: its row's DW tile dwords, then the
-// dwords west and east of the tile (clamped addresses where the tile meets
-// the grid's edge; the masks applied after the load clear what lies outside).
+template <int N, bool MASK, bool RING>
+__device__ __forceinline__ void fine_generations(uint32_t (&s)[N], int gens, uint32_t rowmask,
+                                                 const uint32_t (&colmask)[N]) {
+    for (int g = 0; g < gens; ++g) fine_generation<N, MASK, RING>(s, rowmask, colmask);
+}
+
 template <int DW>
 struct FineRow {
     uint32_t v[DW + 2];
 };
 
-// Issues the lane's loads for `tile`: no branch around them, so all of them are
-// in flight together, and nothing waits for them here. VEC (rows of a
-// multiple of kFineChunk<DW> dwords, so aligned to one chunk): 16-byte loads
-// (8-byte for 2-dword tiles), each wholly inside the row or, past its end,
-// clamped to its start (x0 + 1 is a multiple of the chunk); else one load
-// per dword.
+// The words a lane reads for one tile: its row's DW tile dwords, then the
+// dwords west and east of the tile. Lane l holds row l - T of the tile's rows:
+// lanes below T read the last T rows of the tile above, lanes from T +
+// tile_rows the first T rows of the tile below. Where such a row or a halo
+// dword lies outside the grid the address is clamped into the tile itself
+// (the masks applied after the load clear what it reads). No branch around
+// the loads, so all of them are in flight together, and nothing waits for
+// them here. The tile's own dwords are one vector load (two for 8 dwords).
 template <int DW>
-constexpr int kFineChunk = DW >= 4 ? 4 : DW;
-
-template <int DW, bool VEC>
 __device__ __forceinline__ void fine_load(const uint32_t* __restrict__ src, const FineGeo& g, int tile,
                                           FineRow<DW>& r) {
-    constexpr int C = kFineChunk<DW>;
     const int lane = threadIdx.x & 63;
     const int tx = tile % g.tiles_x, ty = tile / g.tiles_x;
-    const int x0 = tx * DW - 1;
-    const int y = ty * g.tile_rows - g.halo_rows + lane;
-    const uint32_t* row = src + static_cast<size_t>(y >= 0 && y < g.y_rows ? y : 0) * g.x_dw;
-    if (VEC) {
-#pragma unroll
-        for (int q = 0; q < DW / C; ++q) {
-            const int c = x0 + 1 + C * q;
-            const uint32_t* at = row + (c < g.x_dw ? c : 0);
-            if constexpr (C == 4) {
-                const uint4 w = *reinterpret_cast<const uint4*>(at);
-                r.v[4 * q] = w.x;
-                r.v[4 * q + 1] = w.y;
-                r.v[4 * q + 2] = w.z;
-                r.v[4 * q + 3] = w.w;
-            } else {
-                const uint2 w = *reinterpret_cast<const uint2*>(at);
-                r.v[2 * q] = w.x;
-                r.v[2 * q + 1] = w.y;
-            }
-        }
+    const int k = lane < g.halo_rows ? -1 : (lane < g.halo_rows + g.tile_rows ? 0 : 1);  // tile row ty + k
+    const int row = lane - g.halo_rows - k * g.tile_rows;                                // row in that tile
+    const int tyy = ty + k >= 0 && ty + k < g.tiles_y ? ty + k : ty;
+    const size_t tile_dw = static_cast<size_t>(g.tile_rows) * DW;
+    const uint32_t* line = src + static_cast<size_t>(tyy) * g.tiles_x * tile_dw + static_cast<size_t>(row) * DW;
+    const uint32_t* own = line + tx * tile_dw;
+    if constexpr (DW == 2) {
+        const uint2 w = *reinterpret_cast<const uint2*>(own);
+        r.v[0] = w.x;
+        r.v[1] = w.y;
     } else {
 #pragma unroll
-        for (int d = 0; d < DW; ++d) r.v[d] = row[x0 + 1 + d < g.x_dw ? x0 + 1 + d : 0];
+        for (int q = 0; q < DW / 4; ++q) {
+            const uint4 w = reinterpret_cast<const uint4*>(own)[q];
+            r.v[4 * q] = w.x;
+            r.v[4 * q + 1] = w.y;
+            r.v[4 * q + 2] = w.z;
+            r.v[4 * q + 3] = w.w;
+        }
     }
-    r.v[DW] = row[x0 >= 0 ? x0 : 0];
-    r.v[DW + 1] = row[x0 + DW + 1 < g.x_dw ? x0 + DW + 1 : 0];
+    r.v[DW] = line[(tx > 0 ? tx - 1 : tx) * tile_dw + DW - 1];
+    r.v[DW + 1] = line[(tx + 1 < g.tiles_x ? tx + 1 : tx) * tile_dw];
 }
 
 // Advances one tile, whose words fine_load read, by gens generations. With
@@ -398,7 +402,7 @@ __device__ __forceinline__ void fine_load(const uint32_t* __restrict__ src, cons
 // device-scope atomics that one counter would take one after another, over
 // kFineLists counters in lines of their own. The wavefront clears the mark
 // of a listed tile it advances (its list is done with it).
-template <int DW, bool RING, bool VEC>
+template <int DW, bool RING>
 __device__ __forceinline__ void fine_tile(const FineRow<DW>& r, uint32_t* __restrict__ dst, const FineGeo& g,
                                           int gens, int tile, bool listed, const FineTrack& tr) {
     constexpr int N = RING ? DW + 1 : DW + 2;  // s[0]: (west) halo, s[1..DW]: the tile, s[DW + 1]: east halo
@@ -439,22 +443,15 @@ __device__ __forceinline__ void fine_tile(const FineRow<DW>& r, uint32_t* __rest
 #pragma unroll
     for (int d = 0; d < DW; ++d) diff |= before[d] != s[1 + d];
     if (own) {
-        uint32_t* out = dst + static_cast<size_t>(y) * g.x_dw;
-        if (VEC) {
-            constexpr int C = kFineChunk<DW>;
-#pragma unroll
-            for (int q = 0; q < DW / C; ++q) {
-                if (!colmask[1 + C * q]) continue;
-                if constexpr (C == 4)
-                    *reinterpret_cast<uint4*>(out + x0 + 1 + 4 * q) =
-                        make_uint4(s[1 + 4 * q], s[2 + 4 * q], s[3 + 4 * q], s[4 + 4 * q]);
-                else
-                    *reinterpret_cast<uint2*>(out + x0 + 1 + 2 * q) = make_uint2(s[1 + 2 * q], s[2 + 2 * q]);
-            }
+        // the tile's row, padding dwords past the grid's width included (the
+        // masks keep them 0)
+        uint32_t* out = dst + (static_cast<size_t>(tile) * g.tile_rows + (lane - g.halo_rows)) * DW;
+        if constexpr (DW == 2) {
+            *reinterpret_cast<uint2*>(out) = make_uint2(s[1], s[2]);
         } else {
 #pragma unroll
-            for (int j = 1; j <= DW; ++j)
-                if (colmask[j]) out[x0 + j] = s[j];
+            for (int q = 0; q < DW / 4; ++q)
+                reinterpret_cast<uint4*>(out)[q] = make_uint4(s[1 + 4 * q], s[2 + 4 * q], s[3 + 4 * q], s[4 + 4 * q]);
         }
     }
     if (!tr.list_next) return;
@@ -490,7 +487,7 @@ __device__ __forceinline__ void fine_tile(const FineRow<DW>& r, uint32_t* __rest
 // of the grid is active). The entry after it is read before the tile
 // advances, a scalar load the wavefront waits for. The launch resets the
 // counters of the list after next.
-template <int DW, bool RING, bool VEC>
+template <int DW, bool RING>
 __global__ __launch_bounds__(64 * kFineWaves) void fine_tile_kernel(const uint32_t* __restrict__ src,
                                                                     uint32_t* __restrict__ dst, FineGeo g, int gens,
                                                                     const int* __restrict__ list,
@@ -527,7 +524,7 @@ __global__ __launch_bounds__(64 * kFineWaves) void fine_tile_kernel(const uint32
     int tile = listed ? first : at;
     int next_tile = at + step < n ? tile_at(at + step) : -1;
     FineRow<DW> cur;
-    fine_load<DW, VEC>(src, g, tile, cur);
+    fine_load<DW>(src, g, tile, cur);
     for (;;) {
         // the words arrive here, not earlier: nothing the compiler would hoist
         // to the loads that fetch them makes the wavefront wait for them there
@@ -536,14 +533,14 @@ __global__ __launch_bounds__(64 * kFineWaves) void fine_tile_kernel(const uint32
         FineRow<DW> nxt;
         int after = -1;
         if (next_tile >= 0) {
-            fine_load<DW, VEC>(src, g, next_tile, nxt);
+            fine_load<DW>(src, g, next_tile, nxt);
             if (at + 2 * step < n) after = tile_at(at + 2 * step);
         }
         // keeps the loads here, issued before the tile advances (the compiler
         // would otherwise sink them to their use); it reads no loaded word, so
         // nothing waits for them
         asm volatile("" ::: "memory");
-        fine_tile<DW, RING, VEC>(cur, dst, g, gens, tile, listed != 0, tr);
+        fine_tile<DW, RING>(cur, dst, g, gens, tile, listed != 0, tr);
         if (next_tile < 0) return;
         at += step;
         tile = next_tile;
@@ -1132,17 +1129,13 @@ int tile_occupancy(const TileShape& shape) {
     return per_cu;
 }
 
-// The change-tracking kernel for tiles of dw dwords, the ring form or not,
-// vector loads or not.
+// The change-tracking kernel for tiles of dw dwords, the ring form or not.
 template <int DW>
-static auto fine_kernel_for(bool ring, bool vec) {
-    using K = decltype(&fine_tile_kernel<DW, true, true>);
-    if (ring) return vec ? K(&fine_tile_kernel<DW, true, true>) : K(&fine_tile_kernel<DW, true, false>);
-    return vec ? K(&fine_tile_kernel<DW, false, true>) : K(&fine_tile_kernel<DW, false, false>);
+static auto fine_kernel_for(bool ring) {
+    return ring ? &fine_tile_kernel<DW, true> : &fine_tile_kernel<DW, false>;
 }
 
 static bool fine_ring(int halo_rows) { return halo_rows <= kFineRingMaxSteps; }
-static bool fine_vec(int64_t x_dw, int tile_dwords) { return x_dw % (tile_dwords >= 4 ? 4 : tile_dwords) == 0; }
 
 hipError_t launch_fine_tiles(const FineGeo& g, const uint32_t* src, uint32_t* dst, int gens, const int* list,
                              const int* count, bool listed, const FineTrack& track, hipStream_t stream) {
@@ -1154,30 +1147,67 @@ hipError_t launch_fine_tiles(const FineGeo& g, const uint32_t* src, uint32_t* ds
     const dim3 grid(static_cast<unsigned>(fine_workgroups(g))), block(64 * kFineWaves);
     FineTrack t = track;
     t.list_cap = fine_list_cap(g);
-    const bool ring = fine_ring(g.halo_rows), vec = fine_vec(g.x_dw, g.tile_dwords);
+    const bool ring = fine_ring(g.halo_rows);
     if (g.tile_dwords == 2)
-        hipLaunchKernelGGL(fine_kernel_for<2>(ring, vec), grid, block, 0, stream, src, dst, g, gens, list, count,
+        hipLaunchKernelGGL(fine_kernel_for<2>(ring), grid, block, 0, stream, src, dst, g, gens, list, count,
                            listed ? 1 : 0, n, t);
     else if (g.tile_dwords == 4)
-        hipLaunchKernelGGL(fine_kernel_for<4>(ring, vec), grid, block, 0, stream, src, dst, g, gens, list, count,
+        hipLaunchKernelGGL(fine_kernel_for<4>(ring), grid, block, 0, stream, src, dst, g, gens, list, count,
                            listed ? 1 : 0, n, t);
     else
-        hipLaunchKernelGGL(fine_kernel_for<8>(ring, vec), grid, block, 0, stream, src, dst, g, gens, list, count,
+        hipLaunchKernelGGL(fine_kernel_for<8>(ring), grid, block, 0, stream, src, dst, g, gens, list, count,
                            listed ? 1 : 0, n, t);
     return hipGetLastError();
 }
 
-int fine_resident_workgroups(int tile_dwords, int halo_rows, int64_t x_dw) {
-    const bool ring = fine_ring(halo_rows), vec = fine_vec(x_dw, tile_dwords);
-    const void* f = tile_dwords == 2   ? reinterpret_cast<const void*>(fine_kernel_for<2>(ring, vec))
-                    : tile_dwords == 4 ? reinterpret_cast<const void*>(fine_kernel_for<4>(ring, vec))
-                                       : reinterpret_cast<const void*>(fine_kernel_for<8>(ring, vec));
+int fine_resident_workgroups(int tile_dwords, int halo_rows) {
+    const bool ring = fine_ring(halo_rows);
+    const void* f = tile_dwords == 2   ? reinterpret_cast<const void*>(fine_kernel_for<2>(ring))
+                    : tile_dwords == 4 ? reinterpret_cast<const void*>(fine_kernel_for<4>(ring))
+                                       : reinterpret_cast<const void*>(fine_kernel_for<8>(ring));
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 64 * kFineWaves, 0) != hipSuccess)
         return 0;
     return cus * per_cu;
+}
+
+// Row-major grid <-> the change-tracking kernel's tile-major words. Packing
+// walks the padded extent (whole tiles) in row order, so its reads are
+// coalesced, and writes 0 where the tiles reach past the grid; unpacking
+// walks the grid in row order, so its writes are.
+__global__ __launch_bounds__(256) void fine_pack_kernel(const uint32_t* __restrict__ rows,
+                                                        uint32_t* __restrict__ tiles, FineGeo g) {
+    const int64_t width = static_cast<int64_t>(g.tiles_x) * g.tile_dwords;
+    const int64_t total = width * g.tiles_y * g.tile_rows;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const int64_t y = i / width;
+        const int x = static_cast<int>(i - y * width);
+        tiles[fine_word_index(g, x, y)] = y < g.y_rows && x < g.x_dw ? rows[y * g.x_dw + x] : 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void fine_unpack_kernel(const uint32_t* __restrict__ tiles,
+                                                          uint32_t* __restrict__ rows, FineGeo g) {
+    const int64_t total = static_cast<int64_t>(g.x_dw) * g.y_rows;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const int64_t y = i / g.x_dw;
+        const int x = static_cast<int>(i - y * g.x_dw);
+        rows[i] = tiles[fine_word_index(g, x, y)];
+    }
+}
+
+hipError_t launch_fine_layout(const FineGeo& g, const uint32_t* from, uint32_t* to, bool to_tiles,
+                              hipStream_t stream) {
+    const dim3 grid(2048), block(256);
+    if (to_tiles)
+        hipLaunchKernelGGL(fine_pack_kernel, grid, block, 0, stream, from, to, g);
+    else
+        hipLaunchKernelGGL(fine_unpack_kernel, grid, block, 0, stream, from, to, g);
+    return hipGetLastError();
 }
 
 bool narrow_shape_for(int64_t x_dw, int64_t y_rows, TileShape* s) {

@@ -117,10 +117,29 @@ struct FineGeo {
 // Workgroups of a change-tracking launch (kFineWaves wavefronts each).
 inline int fine_workgroups(const FineGeo& g) { return g.workgroups; }
 
-// Workgroups the change-tracking kernel for tiles of tile_dwords, blocks of
-// halo_rows generations and rows of x_dw dwords can hold resident at once on
-// the current device (0 if unknown).
-int fine_resident_workgroups(int tile_dwords, int halo_rows, int64_t x_dw);
+// The change-tracking kernel keeps the grid tile-major: tile (tx, ty) is
+// tile_rows x tile_dwords dwords, row-major, at dword (ty * tiles_x + tx) *
+// tile_rows * tile_dwords; the tiles past the grid's right and bottom edges
+// are padded with zero dwords. Index of dword x of row y.
+__host__ __device__ inline size_t fine_word_index(const FineGeo& g, int x, int64_t y) {
+    const int tx = x / g.tile_dwords, ty = static_cast<int>(y / g.tile_rows);
+    const int r = static_cast<int>(y - static_cast<int64_t>(ty) * g.tile_rows);
+    return ((static_cast<size_t>(ty) * g.tiles_x + tx) * g.tile_rows + r) * g.tile_dwords + (x - tx * g.tile_dwords);
+}
+// Dwords of the tile-major grid.
+inline size_t fine_words(const FineGeo& g) {
+    return static_cast<size_t>(g.tiles_x) * g.tiles_y * g.tile_rows * g.tile_dwords;
+}
+
+// Workgroups the change-tracking kernel for tiles of tile_dwords and blocks of
+// halo_rows generations can hold resident at once on the current device (0 if
+// unknown).
+int fine_resident_workgroups(int tile_dwords, int halo_rows);
+
+// Converts a row-major grid (g.y_rows rows of g.x_dw dwords) to the
+// tile-major words (to_tiles) or back.
+hipError_t launch_fine_layout(const FineGeo& g, const uint32_t* from, uint32_t* to, bool to_tiles,
+                              hipStream_t stream);
 
 // A block's list of tiles is kFineLists sub-lists, one per XCD: sub-list j
 // is taken by the workgroups b with b % kFineLists == j, which run on XCD j.

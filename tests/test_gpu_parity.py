@@ -218,6 +218,27 @@ def test_persistent_slab_shapes_against_oracle(band_rows, waves, steps):
         assert np.array_equal(g.download(), ref)
 
 
+@pytest.mark.parametrize("x,y,word_bits,band_rows,steps", [
+    (8224, 900, 32, 9, 4),    # 257 dwords: an odd row width in 32-bit words
+    (4096, 1300, 64, 9, 3),
+    (12352, 777, 64, 10, 8),
+    (16320, 2000, 64, 9, 1),  # one generation per block: every block exchanges
+])
+def test_persistent_slabs_masked_columns(x, y, word_bits, band_rows, steps):
+    """The slab launch on rows narrower than a wavefront's 512 dwords (the
+    column-masked variant) for the 72- and 80-row slabs, against the oracle."""
+    gens = 6 * steps + 1
+    note("persistent masked", x, y, word_bits, band_rows, steps, gens)
+    cells = ol.bernoulli_cells(x, y, x + steps, 0.45)
+    ref = ol.naive_run(cells, gens)
+    with golhip.Grid(x, y, word_bits=word_bits, kernel=golhip.KERNEL_TEMPORAL, temporal_steps=steps,
+                     lane_dwords=8, band_rows=band_rows, waves=8) as g:
+        assert g.info().persistent == 1
+        g.upload(ol.pack(cells, word_bits))
+        g.step(gens)
+        assert np.array_equal(ol.unpack(g.download(), x, y, word_bits), ref)
+
+
 def test_persistent_repeated_launches_match_fixture():
     """Ten persistent launches of 10000 generations on one handle, the input
     coming alternately from a device buffer and from the host (the benchmark's
