@@ -395,10 +395,13 @@ __device__ __forceinline__ void fine_load(const uint32_t* __restrict__ src, cons
 }
 
 // Advances one tile, whose words fine_load read, by gens generations. With
-// tracking (tr.list_next set), a tile whose cells changed appends the tiles of
-// its 3 x 3 neighbourhood to the next block's list: a tile joins once, when
-// its mark goes from 0 to 1, to sub-list fine_sub_list(nx, ny, DW) of the new
-// tile, whose counter reserves its entry. Sub-lists spread the reservations,
+// tracking (tr.list_next set), a tile whose cells changed appends itself and
+// each of its eight neighbours that a changed cell lies within T cells of
+// (Chebyshev distance, T = halo_rows, the generations of a tracked block: the
+// neighbour's next block sees exactly the cells within T of it) to the next
+// block's list: a tile joins once, when its mark goes from 0 to 1, to
+// sub-list fine_sub_list(nx, ny, DW) of the new tile, whose counter reserves
+// its entry. Sub-lists spread the reservations,
 // device-scope atomics that one counter would take one after another, over
 // kFineLists counters in lines of their own. The wavefront clears the mark
 // of a listed tile it advances (its list is done with it).
@@ -442,6 +445,12 @@ __device__ __forceinline__ void fine_tile(const FineRow<DW>& r, uint32_t* __rest
     bool diff = false;
 #pragma unroll
     for (int d = 0; d < DW; ++d) diff |= before[d] != s[1 + d];
+    // changes within reach (halo_rows cells, the generations of a tracked
+    // block) of the west and east neighbours: cells 0 .. reach - 1 of the
+    // first dword, the last reach cells of the last one
+    const int reach = g.halo_rows;
+    const bool west_diff = ((before[0] ^ s[1]) & ((1u << reach) - 1u)) != 0u;
+    const bool east_diff = ((before[DW - 1] ^ s[DW]) >> (32 - reach)) != 0u;
     if (own) {
         // the tile's row, padding dwords past the grid's width included (the
         // masks keep them 0)
@@ -460,8 +469,19 @@ __device__ __forceinline__ void fine_tile(const FineRow<DW>& r, uint32_t* __rest
         atomicAdd(&tr.wg_active[blockIdx.x], 1ull);
     }
     if (__ballot(own && diff) == 0) return;
-    // lanes 0..8: the 3 x 3 neighbourhood joins the next block's list
-    if (lane < 9) {
+    // A neighbour joins the next block's list only if a cell within its
+    // reach changed (the cells it sees over the next block); the tile itself
+    // joins if any of its cells changed. Bit 3 (dy + 1) + dx + 1 of `near`:
+    // the neighbour at (tx + dx, ty + dy).
+    const int row = lane - g.halo_rows;
+    const bool high = row < reach, low = row >= g.tile_rows - reach;  // within reach of the tile above / below
+    const bool west = own && west_diff, east = own && east_diff;
+    const unsigned near = (__ballot(west && high) != 0 ? 1u : 0u) | (__ballot(own && diff && high) != 0 ? 2u : 0u) |
+                          (__ballot(east && high) != 0 ? 4u : 0u) | (__ballot(west) != 0 ? 8u : 0u) | 16u |
+                          (__ballot(east) != 0 ? 32u : 0u) | (__ballot(west && low) != 0 ? 64u : 0u) |
+                          (__ballot(own && diff && low) != 0 ? 128u : 0u) | (__ballot(east && low) != 0 ? 256u : 0u);
+    // lanes 0..8: the neighbours within reach join the next block's list
+    if (lane < 9 && ((near >> lane) & 1u)) {
         const int nx = tx + lane % 3 - 1, ny = ty + lane / 3 - 1;
         if (nx >= 0 && nx < g.tiles_x && ny >= 0 && ny < g.tiles_y) {
             const int n = ny * g.tiles_x + nx;

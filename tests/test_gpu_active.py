@@ -157,6 +157,77 @@ def test_pulsars_quiet_at_12_listed_at_16():
     assert info16.active_tiles_total >= info16.tiles + n * (info16.tracked_blocks - 1)
 
 
+GLIDER_SE = np.array([[0, 1, 0], [0, 0, 1], [1, 1, 1]], np.uint8)
+
+
+def listed_tiles_model(cells, gens, steps, dw):
+    """Tiles a tracked run advances, by the listing rule stepped on the CPU: the
+    first block advances every tile; a later block advances a tile that changed
+    over the block before, and each neighbour of a changed tile that one of its
+    changed cells lies within `steps` cells of (Chebyshev distance). The blocks'
+    states come from the oracle's bit-parallel engine."""
+    y, x = cells.shape
+    tw, tr = 32 * dw, 64 - 2 * steps
+    ty, tx = -(-y // tr), -(-x // tw)
+    words = ol.pack(cells, 64) if x % 64 == 0 else None
+    prev = cells
+    total = tx * ty
+    for b in range(gens // steps - 1):
+        cur = (ol.unpack(ol.bitpar_run(ol.pack(prev, 64), x, y, steps), x, y, 64) if words is not None
+               else ol.naive_run(prev, steps))
+        d = np.zeros((ty * tr, tx * tw), bool)
+        d[:y, :x] = prev != cur
+        d = d.reshape(ty, tr, tx, tw)
+        r = steps
+        anyc = d.any((1, 3))
+        part = {(dy, dx): d[:, rows, :, cols].any((1, 3))
+                for dy, rows in ((-1, slice(0, r)), (0, slice(None)), (1, slice(tr - r, tr)))
+                for dx, cols in ((-1, slice(0, r)), (0, slice(None)), (1, slice(tw - r, tw)))}
+        listed = anyc.copy()
+        for (dy, dx), m in part.items():
+            # a change in region (dy, dx) of tile (i, j) lists tile (i + dy, j + dx)
+            src = m[max(0, -dy):ty - max(0, dy), max(0, -dx):tx - max(0, dx)]
+            listed[max(0, dy):ty + min(0, dy), max(0, dx):tx + min(0, dx)] |= src
+        total += int(listed.sum())
+        prev = cur
+    return total
+
+
+# The listing rule is exact and tight: gliders crossing tile corners in all
+# four directions, blinkers and random patches near tile edges, at block
+# depths with both halo forms and every tile width; the tiles the GPU advanced
+# equal the CPU model's count, and the result equals the naive oracle's.
+@pytest.mark.parametrize("x,y,steps,dw", [
+    (2112, 300, 16, 2), (2048, 257, 9, 2), (2048, 300, 21, 2), (2176, 200, 12, 4), (2048, 129, 5, 8),
+])
+def test_listing_follows_reach(x, y, steps, dw):
+    rng = np.random.default_rng(x + y + steps)
+    c = np.zeros((y, x), np.uint8)
+    tw, tr = 32 * dw, 64 - 2 * steps
+    for k in range(24):
+        gy, gx = int(rng.integers(0, y - 8)), int(rng.integers(0, x - 8))
+        g = GLIDER_SE
+        if k & 1:
+            g = g[:, ::-1]
+        if k & 2:
+            g = g[::-1, :]
+        c[gy:gy + 3, gx:gx + 3] |= g
+    for k in range(16):  # blinkers hugging tile edges (period 2)
+        ty, tx = int(rng.integers(0, -(-y // tr))), int(rng.integers(0, -(-x // tw)))
+        yy = min(ty * tr + int(rng.choice([0, 1, steps - 1, steps, tr - steps - 1, tr - 2])), y - 1)
+        xx = min(tx * tw + int(rng.choice([0, steps - 2, steps - 1, steps, tw - steps - 1, tw - 3])), x - 3)
+        c[yy, xx:xx + 3] = 1
+    for k in range(6):  # small random soups
+        gy, gx = int(rng.integers(0, y - 12)), int(rng.integers(0, x - 12))
+        c[gy:gy + 12, gx:gx + 12] |= (rng.random((12, 12)) < 0.4).astype(np.uint8)
+    gens = 14 * steps
+    note("active listing", x, y, "steps", steps, "lane_dwords", dw, "gens", gens)
+    out, info = run(ol.pack(c), x, y, gens, temporal_steps=steps, lane_dwords=dw)
+    assert np.array_equal(ol.unpack(out, x, y), ol.naive_run(c, gens))
+    assert info.tracked_blocks == 14
+    assert info.active_tiles_total == listed_tiles_model(c, gens, steps, dw)
+
+
 def test_split_calls_and_reupload():
     x, y = 2048, 600
     w = golhip.fill_patches(x, y, 11, 6, 64, 0.35)
