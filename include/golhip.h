@@ -58,9 +58,10 @@ enum gol_kernel {
                                 (reference: cuda-temporal-rowed/temporal_rowed.cu:33-229) */
     GOL_KERNEL_ACTIVE = 3    /* change tracking over a device-side active-tile list: a tile of
                                 32 * lane_dwords cells (lane_dwords 2, 4 or 8, default 2) x
-                                (64 - 2 * temporal_steps) rows is advanced only when it or a
-                                neighbour changed over the last block of temporal_steps (1..21,
-                                default 16) generations; band_rows and waves do not apply
+                                (64 - 2 * temporal_steps) rows is advanced only when a cell
+                                within temporal_steps cells of it changed over the last block of
+                                temporal_steps (1..21, default 16) generations; band_rows and
+                                waves do not apply
                                 (reference: cuda-naive-local-one-cell/cuda_local_one_cell.cu:53-242) */
 };
 

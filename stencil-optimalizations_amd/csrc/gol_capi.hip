@@ -394,12 +394,14 @@ int enqueue_persistent(gol_handle* h, uint64_t generations) {
 
 // One change-tracking block of h->steps generations, one launch: the tiles
 // on this block's list (every tile when the lists are not valid) advance, and
-// each tile that changed puts its 3 x 3 neighbourhood on the next block's
-// list. A skipped tile keeps the state it had two blocks ago in the output
-// buffer, which equals its current state: it and its neighbours did not
-// change over the last block, so the block starting now reproduces the same
-// state (the tiles are at least `steps` cells wide and tall, the reach of
-// `steps` generations). Lists, marks and counters rotate over three slots:
+// each tile that changed puts itself and the neighbours within `steps` cells
+// of a changed cell on the next block's list. A skipped tile keeps the state
+// it had two blocks ago in the output buffer, which equals its current state:
+// no cell within reach of it changed over the last block, so the block
+// starting now reproduces the same state (the tiles are at least `steps`
+// cells wide and tall, the reach of `steps` generations, so that reach lies
+// in the tile and its eight neighbours). Lists, marks and counters rotate
+// over three slots:
 // block b reads slot b % 3, appends to slot (b + 1) % 3 and resets the
 // counter of slot (b + 2) % 3, which block b - 1 read.
 int enqueue_active_block(gol_handle* h) {

@@ -178,8 +178,9 @@ struct FineTrack {
 
 // One change-tracking block: advance the tiles of list's sub-lists (lengths
 // in count) when listed, else every tile, by gens (<= halo_rows)
-// generations; with track set, the tiles whose cells changed append their
-// 3 x 3 neighbourhoods to the next list. list (kFineLists * track.list_cap
+// generations; with track set, the tiles whose cells changed append
+// themselves and the neighbours a changed cell lies within halo_rows cells
+// of to the next list. list (kFineLists * track.list_cap
 // entries) and count (kFineLists counters) are valid device pointers either
 // way: the kernel reads both before it knows which it uses.
 hipError_t launch_fine_tiles(const FineGeo& g, const uint32_t* src, uint32_t* dst, int gens, const int* list,
