@@ -145,8 +145,10 @@ struct gol_server {
     std::vector<int> free_slots;
     struct Lane {
         hipStream_t stream = nullptr;
-        WorkItem* d_items[2] = {nullptr, nullptr};
-        WorkItem* h_items[2] = {nullptr, nullptr};  // pinned staging
+        // a round's items, double-buffered, in pinned host memory that the
+        // kernels read directly (coherent, mapped): no copy per round
+        WorkItem* h_items[2] = {nullptr, nullptr};
+        WorkItem* dev_items[2] = {nullptr, nullptr};  // the same memory as the device addresses it
         size_t cap = 0;
         hipEvent_t round_ev[2] = {nullptr, nullptr};
         bool used[2] = {false, false};
@@ -201,13 +203,12 @@ int ensure_item_capacity(gol_server::Lane& L, size_t n) {
     if ((e = hipStreamSynchronize(L.stream)) != hipSuccess) return fail_hip(e, "server lane");
     size_t cap = std::max<size_t>(n, 2 * L.cap);
     for (int k = 0; k < 2; ++k) {
-        if (L.d_items[k]) (void)hipFree(L.d_items[k]);
         if (L.h_items[k]) (void)hipHostFree(L.h_items[k]);
-        L.d_items[k] = nullptr;
         L.h_items[k] = nullptr;
-        if ((e = hipMalloc(&L.d_items[k], cap * sizeof(WorkItem))) != hipSuccess) return fail_hip(e, "server items");
-        if ((e = hipHostMalloc(reinterpret_cast<void**>(&L.h_items[k]), cap * sizeof(WorkItem), hipHostMallocDefault)) !=
-            hipSuccess)
+        L.dev_items[k] = nullptr;
+        if ((e = hipHostMalloc(reinterpret_cast<void**>(&L.h_items[k]), cap * sizeof(WorkItem),
+                               hipHostMallocMapped | hipHostMallocCoherent)) != hipSuccess ||
+            (e = hipHostGetDevicePointer(reinterpret_cast<void**>(&L.dev_items[k]), L.h_items[k], 0)) != hipSuccess)
             return fail_hip(e, "server items (host)");
     }
     L.cap = cap;
@@ -340,11 +341,10 @@ bool schedule_once(gol_server* s) {
                 std::memcpy(L.h_items[k] + at, kv.second.items.data(), kv.second.items.size() * sizeof(WorkItem));
                 at += kv.second.items.size();
             }
-            // new jobs' uploads, then the items (a few KB, on the lane itself)
-            if ((lane_waits_upload[ln] && (e = hipStreamWaitEvent(L.stream, s->up_ev, 0)) != hipSuccess) ||
-                (e = hipMemcpyAsync(L.d_items[k], L.h_items[k], at * sizeof(WorkItem), hipMemcpyHostToDevice,
-                                    L.stream)) != hipSuccess) {
-                s->worker_error = fail_hip(e, "server items upload");
+            // the lane's round follows new jobs' uploads; its kernels read the
+            // items from host memory, written above before the launches
+            if (lane_waits_upload[ln] && (e = hipStreamWaitEvent(L.stream, s->up_ev, 0)) != hipSuccess) {
+                s->worker_error = fail_hip(e, "server upload order");
                 return false;
             }
             at = 0;
@@ -352,7 +352,7 @@ bool schedule_once(gol_server* s) {
                 const LaunchKey& key = kv.first;
                 const TileShape sh{key.lane_dwords, key.band_rows, key.band_rows ? key.waves : kv.second.waves};
                 const int n = static_cast<int>(kv.second.items.size());
-                if ((e = launch_items(sh, s->d_desc, L.d_items[k] + at, n, L.stream)) != hipSuccess) {
+                if ((e = launch_items(sh, s->d_desc, L.dev_items[k] + at, n, L.stream)) != hipSuccess) {
                     s->worker_error = fail_hip(e, "server round launch");
                     return false;
                 }
@@ -458,7 +458,6 @@ void destroy_resources(gol_server* s) {
     sync_streams(s);
     for (auto& L : s->lanes) {
         for (int k = 0; k < 2; ++k) {
-            if (L.d_items[k]) (void)hipFree(L.d_items[k]);
             if (L.h_items[k]) (void)hipHostFree(L.h_items[k]);
             if (L.round_ev[k]) (void)hipEventDestroy(L.round_ev[k]);
         }
