@@ -144,6 +144,56 @@ def leg_mostly_stable() -> dict:
     }
 
 
+# The reference's published change-tracking measurements on H100 ("Optimized Tiled 64-bit",
+# 16384^2 x 10000, cluster-experiments/final-measurements/hopper/job-144324.out:1603,1736,1869,2002):
+# cell-updates/s per workload. Other hardware, another kernel: reported beside ours, not a target.
+REFERENCE_CT_H100 = {"zeros-16384": 4.19e12, "always-changing-16384": 1.36e12,
+                     "spacefiller-33-work-16384": 2.45e12, "spacefiller-66-work-16384": 1.79e12}
+
+
+def reference_ct_input(f):
+    """Inputs of the reference's change-tracking workloads (generate-final-measurements.py:21, 27-30)."""
+    x, y = f["x"], f["y"]
+    if f["generator"] == "zeros":
+        return np.zeros(x * y // 64, np.uint64)
+    if f["generator"] == "always-changing":
+        # AlwaysChangingSpaceLoader (data_loader.hpp:117-134): a horizontal blinker every 4th column and row
+        c = np.zeros((y, x), np.uint8)
+        for k in range(3):
+            c[0::4, k::4] = 1
+        bits = np.packbits(c, axis=1, bitorder="little")
+        return np.ascontiguousarray(bits).view(np.uint64).reshape(-1)
+    return golhip.fill_lexicon(x, y, f["params"]["expression"])
+
+
+def leg_reference_workloads() -> dict:
+    """Change tracking on the reference's four change-tracking workloads (no work, full work,
+    33 % and 66 % busy; 16384^2 x 10000), each checked against the oracle fixture, with the
+    dense persistent launch on the same input beside it."""
+    gens = 10000
+    out = {}
+    for name, ref_rate in REFERENCE_CT_H100.items():
+        f = fixture(name, gens)
+        w = reference_ct_input(f)
+        row = {"reference_h100_cell_updates_per_s": ref_rate}
+        for label, kernel in (("change_tracking", golhip.KERNEL_ACTIVE), ("dense", golhip.KERNEL_TEMPORAL)):
+            with golhip.Grid(X, Y, kernel=kernel, device=0) as g:
+                g.upload(w)
+                g.step_timed(24)  # warm-up
+                g.upload(w)
+                ms = g.step_timed(gens)
+                exact = hashlib.sha256(g.download().tobytes()).hexdigest() == f["sha256"]
+            if not exact:
+                raise SystemExit(f"reference workload {name}: the {label} result differs from the oracle fixture")
+            row[label] = {"ms": ms, "cell_updates_per_s": X * Y * gens / (ms / 1e3), "bit_exact_vs_oracle": exact}
+        row["change_tracking_vs_reference_h100"] = row["change_tracking"]["cell_updates_per_s"] / ref_rate
+        out[name] = row
+    out["note"] = ("the reference's rates are its published H100 measurements of its own change-tracking kernel "
+                   "(another GPU); blinkers have period 2, so with 16-generation blocks the full-work grid is "
+                   "quiet after its first block here, where the reference's per-generation tracking keeps it busy")
+    return out
+
+
 TUNE_SIDES = (64, 128, 256, 512, 1024, 2048)
 
 
@@ -448,8 +498,8 @@ def main() -> int:
     ap.add_argument("--kernel", default="temporal", choices=["temporal", "plain"])
     ap.add_argument("--cpu-sample-generations", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--legs", default="mostly_stable,spacefiller,batched,served",
-                    help="extra workloads reported beside the headline: mostly_stable,spacefiller,batched,served")
+    ap.add_argument("--legs", default="mostly_stable,spacefiller,reference_workloads,batched,served",
+                    help="extra workloads reported beside the headline: mostly_stable,spacefiller,reference_workloads,batched,served")
     ap.add_argument("--batch-grids", type=int, default=4096)
     ap.add_argument("--plain-sample", type=int, default=48)
     ap.add_argument("--served-jobs", type=int, default=400)
@@ -597,6 +647,9 @@ def main() -> int:
     if "spacefiller" in legs:
         log("spacefiller leg ...")
         result["spacefiller"] = leg_spacefiller()
+    if "reference_workloads" in legs:
+        log("reference change-tracking workloads leg ...")
+        result["reference_workloads"] = leg_reference_workloads()
     if "batched" in legs:
         log("autotuning batch size classes ...")
         result["autotune"] = autotune(not args.no_autotune)

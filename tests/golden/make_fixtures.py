@@ -54,6 +54,21 @@ NAIVE_CASES = [
     (64, 1536, "bernoulli", 17, 0.5, 64, [1, 40, 150]),
 ]
 
+# spacefillers on a 3 x 3 and a 6 x 5 lattice (generate-final-measurements.py:29-30)
+SPACEFILLERS_33 = "; ".join(f"spacefiller[{x}, {y}]" for x in (4096, 8192, 12288) for y in (4096, 8192, 12288))
+SPACEFILLERS_66 = "; ".join(f"spacefiller[{x}, {y}]" for x in (2340, 4680, 7020, 9360, 11700, 14040)
+                            for y in (2730, 5460, 8190, 10920, 13650))
+
+
+def always_changing_words(x, y):
+    """AlwaysChangingSpaceLoader (data_loader.hpp:117-134): a horizontal blinker
+    (three cells of one row) at every 4th column and row, rows-64 words."""
+    c = np.zeros((y, x), np.uint8)
+    for k in range(3):
+        c[0::4, k::4] = 1
+    return ol.pack(c, 64)
+
+
 BITPAR_CASES = [
     # name, x, y, generator, seed, params, generations list
     ("random-16384", 16384, 16384, "random", 42, {}, [1, 100, 1000, 10000]),
@@ -61,6 +76,13 @@ BITPAR_CASES = [
     ("random-4096x8192", 4096, 8192, "bernoulli", 7, {"p": 0.4}, [500]),
     # the reference's published workload (cluster-experiments/generate-final-measurements.py:33)
     ("spacefiller-16384", 16384, 16384, "lexicon", 0, {"expression": "spacefiller[8192, 8192]"}, [1000, 20000]),
+    # the reference's change-tracking workloads at 16384^2 x 10000
+    # (cluster-experiments/generate-final-measurements.py:21, 27-30): no work, full work (blinkers every
+    # 4 cells), and 9 / 30 spacefillers
+    ("zeros-16384", 16384, 16384, "zeros", 0, {}, [10000]),
+    ("always-changing-16384", 16384, 16384, "always-changing", 0, {}, [10000]),
+    ("spacefiller-33-work-16384", 16384, 16384, "lexicon", 0, {"expression": SPACEFILLERS_33}, [10000]),
+    ("spacefiller-66-work-16384", 16384, 16384, "lexicon", 0, {"expression": SPACEFILLERS_66}, [10000]),
 ]
 
 
@@ -77,6 +99,10 @@ def bitpar_case(name, x, y, gen, seed, params, gens_list):
         w = golhip.fill_bernoulli(x, y, seed, params["p"])
     elif gen == "lexicon":
         w = golhip.fill_lexicon(x, y, params["expression"])
+    elif gen == "zeros":
+        w = np.zeros(x * y // 64, np.uint64)
+    elif gen == "always-changing":
+        w = always_changing_words(x, y)
     else:
         w = golhip.fill_patches(x, y, seed, params["n_patches"], params["patch"], params["p"])
     w0 = w

@@ -242,6 +242,35 @@ def test_split_calls_and_reupload():
         assert np.array_equal(g.download(), ref)
 
 
+def reference_workload_input(f):
+    """The inputs of the reference's change-tracking workloads
+    (cluster-experiments/generate-final-measurements.py:21, 27-30)."""
+    x, y = f["x"], f["y"]
+    if f["generator"] == "zeros":
+        return np.zeros(x * y // 64, np.uint64)
+    if f["generator"] == "always-changing":
+        # AlwaysChangingSpaceLoader (data_loader.hpp:117-134): a horizontal
+        # blinker at every 4th column and row
+        c = np.zeros((y, x), np.uint8)
+        for k in range(3):
+            c[0::4, k::4] = 1
+        return ol.pack(c, 64)
+    return golhip.fill_lexicon(x, y, f["params"]["expression"])
+
+
+# The reference's published change-tracking measurements (no work, full work,
+# 33 % and 66 % busy), each 16384^2 x 10000, bit-exact against the oracle.
+@pytest.mark.parametrize("name", ["zeros-16384", "always-changing-16384", "spacefiller-33-work-16384",
+                                  "spacefiller-66-work-16384"])
+def test_reference_change_tracking_workloads(name):
+    f = [r for r in FIX["bitpar_runs"] if r["name"] == name and r["generations"] == 10000][0]
+    w = reference_workload_input(f)
+    assert ol.sha256(w) == f["sha256_input"]
+    note("active reference workload", name)
+    out, info = run(w, f["x"], f["y"], 10000)
+    assert ol.sha256(out) == f["sha256"]
+
+
 def test_full_size_patches_20000():
     f = [r for r in FIX["bitpar_runs"] if r["name"] == "patches-16384" and r["generations"] == 20000][0]
     p = f["params"]
