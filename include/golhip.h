@@ -57,7 +57,8 @@ enum gol_kernel {
                                 once, a single persistent launch runs every block
                                 (reference: cuda-temporal-rowed/temporal_rowed.cu:33-229) */
     GOL_KERNEL_ACTIVE = 3    /* change tracking over a device-side active-tile list: a tile of
-                                32 * lane_dwords cells (lane_dwords 2, 4 or 8, default 2) x
+                                32 * lane_dwords cells (lane_dwords 2, 4 or 8; 0: 2, moving to 4
+                                while a step's blocks list many tiles, and back) x
                                 (64 - 2 * temporal_steps) rows is advanced only when a cell
                                 within temporal_steps cells of it changed over the last block of
                                 temporal_steps (1..21, default 16) generations; band_rows and
@@ -130,6 +131,8 @@ typedef struct gol_info {
     uint64_t device_bytes;    /* device memory owned by the handle */
     uint32_t native_layout;   /* 1: the layout's own words are advanced (GOL_FLAG_NATIVE_LAYOUT) */
     uint32_t workgroups;      /* GOL_KERNEL_ACTIVE: workgroups per launch (one wavefront per tile) */
+    uint32_t tile_width_switches; /* GOL_KERNEL_ACTIVE with lane_dwords 0: changes between 64- and
+                                     128-cell tiles so far (lane_dwords reports the width in use) */
 } gol_info;
 
 typedef struct gol_handle gol_handle;

@@ -5,11 +5,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/golhip.h"
@@ -166,8 +168,17 @@ struct gol_handle {
     uint64_t launches = 0;
     uint64_t generations = 0;
     // change tracking (GOL_KERNEL_ACTIVE): small tiles, one wavefront each
-    FineGeo fine{};
-    uint32_t* fbuf[2] = {nullptr, nullptr};  // the grid tile-major (fine_word_index), fine_words(fine) dwords each
+    FineGeo fine{};               // the tile geometry in use (fine_geo[fine_sel])
+    FineGeo fine_geo[2]{};        // 64-cell tiles, and 128-cell tiles when the width adapts
+    int fine_sel = 0;
+    bool fine_adapt = false;      // lane_dwords left to the library: the width follows the activity
+    unsigned long long* fine_sample = nullptr;  // [kFineLag][kFineLists] pinned, coherent: tag << 32 | length
+    unsigned long long* fine_sample_dev = nullptr;  // the same memory as the device addresses it
+    int fine_sample_tag[kFineLag] = {};  // the tag each chunk's sample carries (0: no sample)
+    int fine_sample_sel[kFineLag] = {};  // the geometry each sample was taken in
+    int fine_tags = 0;                   // tags handed out so far
+    uint64_t fine_switches = 0;   // tile-width changes so far
+    uint32_t* fbuf[2] = {nullptr, nullptr};  // the grid tile-major (fine_word_index), fine_words dwords each
     int fcur = 0;                 // fbuf[fcur] holds the state while a step runs
     uint32_t* marks = nullptr;    // [3][n_tiles] tile is on list k (lists rotate by block)
     int* lists = nullptr;         // [3][kFineLists][fine_list_cap] active-tile lists
@@ -228,6 +239,24 @@ int validate_config(const gol_config* c) {
 }
 
 // Resolve kernel, tile shape and generations per launch for a handle.
+// Change-tracking tiles of dw dwords for blocks of `steps` generations; the
+// launch holds no more workgroups than the device holds at once (wavefronts
+// loop over list slots).
+FineGeo fine_geometry(int64_t x_dw, int64_t y_rows, int dw, int steps) {
+    FineGeo g{};
+    g.x_dw = static_cast<int>(x_dw);
+    g.y_rows = static_cast<int>(y_rows);
+    g.tile_dwords = dw;
+    g.halo_rows = steps;
+    g.tile_rows = 64 - 2 * steps;
+    g.tiles_x = static_cast<int>((x_dw + dw - 1) / dw);
+    g.tiles_y = static_cast<int>((y_rows + g.tile_rows - 1) / g.tile_rows);
+    const int all = (g.tiles_x * g.tiles_y + kFineWaves - 1) / kFineWaves;
+    const int resident = fine_resident_workgroups(dw, steps);
+    g.workgroups = std::max(resident > 0 ? std::min(all, resident) : all, kFineMinWorkgroups);
+    return g;
+}
+
 int plan(gol_handle* h) {
     const gol_config& c = h->cfg;
     h->kernel = c.kernel == GOL_KERNEL_AUTO ? GOL_KERNEL_TEMPORAL : c.kernel;
@@ -274,19 +303,16 @@ int plan(gol_handle* h) {
             return fail(GOL_ERR_INVALID_ARGUMENT, "lane_dwords %d: change-tracking tiles are 2, 4 or 8 dwords wide",
                         dw);
         h->steps = steps;
-        h->fine.x_dw = static_cast<int>(h->x_dw);
-        h->fine.y_rows = static_cast<int>(h->y_rows);
-        h->fine.tile_dwords = dw;
-        h->fine.halo_rows = steps;
-        h->fine.tile_rows = 64 - 2 * steps;
-        h->fine.tiles_x = static_cast<int>((h->x_dw + dw - 1) / dw);
-        h->fine.tiles_y = static_cast<int>((h->y_rows + h->fine.tile_rows - 1) / h->fine.tile_rows);
+        h->fine_geo[0] = fine_geometry(h->x_dw, h->y_rows, dw, steps);
+        // With lane_dwords left to the library the tile width follows the
+        // grid's activity: 64-cell tiles track sparse activity best, 128-cell
+        // tiles (a fifth less VALU work per cell) win once a block lists many
+        // tiles of large busy regions (enqueue_active).
+        h->fine_adapt = c.lane_dwords == 0 && h->x_dw >= 2 * kFineWideDW;
+        h->fine_geo[1] = fine_geometry(h->x_dw, h->y_rows, h->fine_adapt ? kFineWideDW : dw, steps);
+        h->fine_sel = 0;
+        h->fine = h->fine_geo[0];
         h->n_tiles = h->fine.tiles_x * h->fine.tiles_y;
-        // wavefronts loop over list slots: launch no more than the device
-        // holds at once
-        const int all = (h->n_tiles + kFineWaves - 1) / kFineWaves;
-        const int resident = fine_resident_workgroups(dw, steps);
-        h->fine.workgroups = std::max(resident > 0 ? std::min(all, resident) : all, kFineMinWorkgroups);
         h->shape = TileShape{dw, 1, kFineWaves};
         return GOL_OK;
     }
@@ -404,7 +430,7 @@ int enqueue_persistent(gol_handle* h, uint64_t generations) {
 // over three slots:
 // block b reads slot b % 3, appends to slot (b + 1) % 3 and resets the
 // counter of slot (b + 2) % 3, which block b - 1 read.
-int enqueue_active_block(gol_handle* h) {
+int enqueue_active_block(gol_handle* h, unsigned long long* sample = nullptr, int sample_tag = 0) {
     const size_t n = static_cast<size_t>(h->n_tiles);
     hipError_t e;
     if (!h->flags_valid) {
@@ -422,6 +448,8 @@ int enqueue_active_block(gol_handle* h) {
     tr.count_next = h->counts + k1 * kFineCounts;
     tr.count_clear = h->counts + k2 * kFineCounts;
     tr.wg_active = h->wg_active;
+    tr.sample = sample;
+    tr.sample_tag = sample_tag;
     e = launch_fine_tiles(h->fine, h->fbuf[h->fcur], h->fbuf[h->fcur ^ 1], h->steps, h->lists + k * list_len,
                           h->counts + k * kFineCounts, h->flags_valid, tr, h->stream);
     if (e != hipSuccess) return fail_hip(e, "fine_tile_kernel launch");
@@ -439,16 +467,109 @@ int enqueue_active_block(gol_handle* h) {
 // and the result comes back row-major into buf[cur]. The buffers never hold
 // another state between steps: uploads reset the lists, so the first block
 // after one writes every tile.
+// The length of the list chunk slot buf's sample block advanced, once every
+// sub-list's word carries the block's tag; -1 when no sample is pending. The
+// block is queued chunks ago, so this is a short wait on the GPU's progress;
+// should the words not arrive while the stream has gone idle, or after a long
+// wait, the stream is checked for an error and the sample is skipped (the
+// width stays).
+int fine_read_sample(gol_handle* h, int buf, int64_t* listed) {
+    *listed = -1;
+    const unsigned long long* words = h->fine_sample + buf * kFineLists;
+    const unsigned tag = static_cast<unsigned>(h->fine_sample_tag[buf]);
+    h->fine_sample_tag[buf] = 0;
+    if (tag == 0) return GOL_OK;
+    auto read = [&]() {
+        int64_t sum = 0;
+        for (int j = 0; j < kFineLists; ++j) {
+            const unsigned long long w = __atomic_load_n(words + j, __ATOMIC_ACQUIRE);
+            if (static_cast<unsigned>(w >> 32) != tag) return static_cast<int64_t>(-1);
+            sum += static_cast<unsigned>(w);
+        }
+        return sum;
+    };
+    const auto start = std::chrono::steady_clock::now();
+    for (uint64_t i = 0;; ++i) {
+        if ((*listed = read()) >= 0) return GOL_OK;
+        if ((i & 1023) == 1023) {
+            const hipError_t q = hipStreamQuery(h->stream);
+            if (q != hipSuccess && q != hipErrorNotReady) return fail_hip(q, "change-tracking list sample");
+            if (q == hipSuccess) {  // idle: the words are final
+                *listed = read();
+                return GOL_OK;
+            }
+            if (std::chrono::steady_clock::now() - start > std::chrono::seconds(10)) return GOL_OK;
+        }
+        std::this_thread::yield();
+    }
+}
+
+// Moves the state to the other tile geometry: back to rows, then tile-major
+// in the other width; the next block advances every tile.
+int fine_switch_width(gol_handle* h) {
+    hipError_t e = launch_fine_layout(h->fine, h->fbuf[h->fcur], h->buf[h->cur], false, h->stream);
+    if (e != hipSuccess) return fail_hip(e, "change-tracking layout launch");
+    h->fine_sel ^= 1;
+    h->fine = h->fine_geo[h->fine_sel];
+    h->n_tiles = h->fine.tiles_x * h->fine.tiles_y;
+    if ((e = launch_fine_layout(h->fine, h->buf[h->cur], h->fbuf[h->fcur], true, h->stream)) != hipSuccess)
+        return fail_hip(e, "change-tracking layout launch");
+    h->launches += 2;
+    h->flags_valid = false;
+    h->fine_switches += 1;
+    return GOL_OK;
+}
+
 int enqueue_active(gol_handle* h, uint64_t generations) {
     if (generations == 0) return GOL_OK;
     hipError_t e = launch_fine_layout(h->fine, h->buf[h->cur], h->fbuf[h->fcur], true, h->stream);
     if (e != hipSuccess) return fail_hip(e, "change-tracking layout launch");
     h->launches += 1;
     uint64_t left = generations;
-    while (left >= static_cast<uint64_t>(h->steps)) {
-        int rc = enqueue_active_block(h);
-        if (rc) return rc;
-        left -= static_cast<uint64_t>(h->steps);
+    // Blocks go out in chunks of kFineChunk. With an adapting width, the
+    // last block of each chunk writes the length of the list it advances to
+    // host memory, and before chunk c the host reads chunk c - kFineLag's
+    // (waiting for it leaves the chunks between queued on the GPU), and
+    // switches the width when the list is long for 64-cell tiles or short for
+    // 128-cell ones. Every result is exact either way; only the time depends
+    // on it.
+    for (uint64_t chunk = 0; left >= static_cast<uint64_t>(h->steps); ++chunk) {
+        const int buf = static_cast<int>(chunk % kFineLag);
+        if (h->fine_adapt && chunk >= static_cast<uint64_t>(kFineLag)) {
+            int64_t listed;
+            int rc = fine_read_sample(h, buf, &listed);
+            if (rc) return rc;
+            const int64_t tiles = static_cast<int64_t>(h->n_tiles);
+            const bool sampled_here = h->fine_sample_sel[buf] == h->fine_sel;
+            if (listed >= 0 && sampled_here && h->flags_valid &&
+                ((h->fine_sel == 0 && listed * 100 > tiles * kFineWidePercent) ||
+                 (h->fine_sel == 1 && listed * 100 < tiles * kFineNarrowPercent))) {
+                int rc = fine_switch_width(h);
+                if (rc) return rc;
+            }
+        }
+        bool sampled = false;
+        for (int b = 0; b < kFineChunk && left >= static_cast<uint64_t>(h->steps); ++b) {
+            // the chunk's last block writes its list's length to host memory
+            const bool last = b == kFineChunk - 1 || left < 2 * static_cast<uint64_t>(h->steps);
+            const bool sample = h->fine_adapt && last && h->flags_valid;
+            int tag = 0;
+            if (sample) {
+                // tags do not repeat within 10^9 samples: a word left from an
+                // earlier sample never passes for this one
+                h->fine_tags = h->fine_tags % 1000000000 + 1;
+                tag = h->fine_tags;
+            }
+            int rc = enqueue_active_block(h, sample ? h->fine_sample_dev + buf * kFineLists : nullptr, tag);
+            if (rc) return rc;
+            if (sample) {
+                sampled = true;
+                h->fine_sample_tag[buf] = tag;
+                h->fine_sample_sel[buf] = h->fine_sel;
+            }
+            left -= static_cast<uint64_t>(h->steps);
+        }
+        if (!sampled) h->fine_sample_tag[buf] = 0;
     }
     if (left > 0) {
         // a shorter block runs every tile: an oscillator that returns to its
@@ -561,18 +682,29 @@ int gol_create(const gol_config* config, gol_handle** out) {
         return rc;
     }
     if (h->kernel == GOL_KERNEL_ACTIVE) {
-        const size_t nt = static_cast<size_t>(h->n_tiles);
-        const size_t nwg = static_cast<size_t>(fine_workgroups(h->fine));
-        const size_t list_len = kFineLists * static_cast<size_t>(fine_list_cap(h->fine));
+        // sized for the larger of the two tile geometries
+        const FineGeo& g0 = h->fine_geo[0];
+        const FineGeo& g1 = h->fine_geo[1];
+        const size_t nt = std::max<size_t>(static_cast<size_t>(g0.tiles_x) * g0.tiles_y,
+                                           static_cast<size_t>(g1.tiles_x) * g1.tiles_y);
+        const size_t nwg = static_cast<size_t>(std::max(fine_workgroups(g0), fine_workgroups(g1)));
+        const size_t list_len = kFineLists * static_cast<size_t>(std::max(fine_list_cap(g0), fine_list_cap(g1)));
+        const size_t fwords = std::max(fine_words(g0), fine_words(g1));
         if ((e = hipMalloc(&h->marks, 3 * nt * sizeof(uint32_t))) != hipSuccess ||
             (e = hipMalloc(&h->lists, 3 * list_len * sizeof(int))) != hipSuccess ||
             (e = hipMalloc(&h->counts, 3 * kFineCounts * sizeof(int))) != hipSuccess ||
             (e = hipMalloc(&h->wg_active, nwg * sizeof(unsigned long long))) != hipSuccess ||
             (e = hipMemsetAsync(h->wg_active, 0, nwg * sizeof(unsigned long long), h->stream)) != hipSuccess ||
-            (e = hipMalloc(&h->fbuf[0], fine_words(h->fine) * sizeof(uint32_t))) != hipSuccess ||
-            (e = hipMalloc(&h->fbuf[1], fine_words(h->fine) * sizeof(uint32_t))) != hipSuccess ||
-            (e = hipMemsetAsync(h->fbuf[0], 0, fine_words(h->fine) * sizeof(uint32_t), h->stream)) != hipSuccess ||
-            (e = hipMemsetAsync(h->fbuf[1], 0, fine_words(h->fine) * sizeof(uint32_t), h->stream)) != hipSuccess) {
+            (e = hipMalloc(&h->fbuf[0], fwords * sizeof(uint32_t))) != hipSuccess ||
+            (e = hipMalloc(&h->fbuf[1], fwords * sizeof(uint32_t))) != hipSuccess ||
+            (e = hipMemsetAsync(h->fbuf[0], 0, fwords * sizeof(uint32_t), h->stream)) != hipSuccess ||
+            (e = hipMemsetAsync(h->fbuf[1], 0, fwords * sizeof(uint32_t), h->stream)) != hipSuccess ||
+            (h->fine_adapt &&
+             ((e = hipHostMalloc(reinterpret_cast<void**>(&h->fine_sample),
+                                 kFineLag * kFineLists * sizeof(unsigned long long),
+                                 hipHostMallocMapped | hipHostMallocCoherent)) != hipSuccess ||
+              (e = hipHostGetDevicePointer(reinterpret_cast<void**>(&h->fine_sample_dev), h->fine_sample, 0)) !=
+                  hipSuccess))) {
             rc = fail_hip(e, "gol_create (change tracking state)");
             gol_destroy(h);
             return rc;
@@ -600,6 +732,17 @@ int gol_create(const gol_config* config, gol_handle** out) {
     return GOL_OK;
 }
 
+// A new state: every tile is active in the next block, which (change
+// tracking with an adapting width) starts on 64-cell tiles again.
+void new_state(gol_handle* h) {
+    h->flags_valid = false;
+    if (h->kernel == GOL_KERNEL_ACTIVE && h->fine_sel != 0) {
+        h->fine_sel = 0;
+        h->fine = h->fine_geo[0];
+        h->n_tiles = h->fine.tiles_x * h->fine.tiles_y;
+    }
+}
+
 int gol_upload(gol_handle* h, const void* words, size_t n_words) {
     clear_error();
     if (!h || !words) return fail(GOL_ERR_INVALID_ARGUMENT, "null argument");
@@ -616,7 +759,7 @@ int gol_upload(gol_handle* h, const void* words, size_t n_words) {
     }
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return fail_hip(e, "gol_upload");
-    h->flags_valid = false;  // a new state: every tile is active in the next block
+    new_state(h);
     return GOL_OK;
 }
 
@@ -632,7 +775,7 @@ int gol_upload_device(gol_handle* h, const void* device_words, size_t n_words) {
                              h->buf[h->cur], h->x_dw * 32, h->y_rows, h->stream)
             : hipMemcpyAsync(h->buf[h->cur], device_words, h->n_dwords * 4, hipMemcpyDeviceToDevice, h->stream);
     if (e != hipSuccess) return fail_hip(e, "gol_upload_device");
-    h->flags_valid = false;
+    new_state(h);
     return GOL_OK;
 }
 
@@ -720,7 +863,8 @@ int gol_get_info(gol_handle* h, gol_info* info) {
         info->lane_dwords = h->plain_dw;
         info->temporal_steps = 1;
     } else {
-        info->lane_dwords = static_cast<uint32_t>(h->shape.lane_dwords);
+        info->lane_dwords = static_cast<uint32_t>(h->kernel == GOL_KERNEL_ACTIVE ? h->fine.tile_dwords
+                                                                                 : h->shape.lane_dwords);
         info->band_rows = static_cast<uint32_t>(h->shape.band_rows);
         info->waves = static_cast<uint32_t>(h->shape.waves);
         info->temporal_steps = static_cast<uint32_t>(h->steps);
@@ -735,7 +879,8 @@ int gol_get_info(gol_handle* h, gol_info* info) {
     info->native_layout = h->native ? 1u : 0u;
     if (h->kernel == GOL_KERNEL_ACTIVE) {
         DeviceGuard g(h->device);
-        std::vector<unsigned long long> per_wg(static_cast<size_t>(fine_workgroups(h->fine)));
+        std::vector<unsigned long long> per_wg(
+            static_cast<size_t>(std::max(fine_workgroups(h->fine_geo[0]), fine_workgroups(h->fine_geo[1]))));
         hipError_t e = hipMemcpyAsync(per_wg.data(), h->wg_active, per_wg.size() * sizeof(unsigned long long),
                                       hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -744,9 +889,12 @@ int gol_get_info(gol_handle* h, gol_info* info) {
         for (unsigned long long v : per_wg) total += v;
         info->active_tiles_total = total;
         info->tracked_blocks = h->blocks;
+        info->tile_width_switches = static_cast<uint32_t>(h->fine_switches);
         info->workgroups = static_cast<uint32_t>(per_wg.size());
-        info->device_bytes += static_cast<uint64_t>(h->n_tiles) * 3 * sizeof(uint32_t) +
-                              3ull * kFineLists * fine_list_cap(h->fine) * sizeof(int) +
+        const FineGeo& g0 = h->fine_geo[0];
+        info->device_bytes += static_cast<uint64_t>(g0.tiles_x) * g0.tiles_y * 3 * sizeof(uint32_t) +
+                              3ull * kFineLists * fine_list_cap(g0) * sizeof(int) +
+                              2ull * fine_words(g0) * sizeof(uint32_t) +
                               3ull * kFineCounts * sizeof(int) + per_wg.size() * sizeof(unsigned long long);
     }
     return GOL_OK;
@@ -766,6 +914,7 @@ void gol_destroy(gol_handle* h) {
         if (h->wg_active) (void)hipFree(h->wg_active);
         if (h->fbuf[0]) (void)hipFree(h->fbuf[0]);
         if (h->fbuf[1]) (void)hipFree(h->fbuf[1]);
+        if (h->fine_sample) (void)hipHostFree(h->fine_sample);
         if (h->xbuf) (void)hipFree(h->xbuf);
         if (h->slab_flags) (void)hipFree(h->slab_flags);
         if (h->slab_error) (void)hipFree(h->slab_error);

@@ -539,6 +539,14 @@ __global__ __launch_bounds__(64 * kFineWaves) void fine_tile_kernel(const uint32
     int first = __builtin_amdgcn_readfirstlane(entries[at < tr.list_cap ? at : tr.list_cap - 1]);
     int length = __builtin_amdgcn_readfirstlane(count[sub * kFineCountStride]);
     asm volatile("" : "+s"(first), "+s"(length));
+    // The first workgroup of each sub-list hands its length to the host, with
+    // the block's tag in the same 64-bit store (host memory the device writes
+    // through: no copy, no event and no wait in the stream).
+    if (tr.sample && listed && b < kFineLists && threadIdx.x == 0)
+        __hip_atomic_store(tr.sample + sub,
+                           (static_cast<unsigned long long>(static_cast<unsigned>(tr.sample_tag)) << 32) |
+                               static_cast<unsigned>(length),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     const int n = listed ? length : cap;
     if (at >= n) return;
     int tile = listed ? first : at;

@@ -99,6 +99,15 @@ int tile_occupancy(const TileShape& shape);
 // tile_rows = 64 - 2T.
 constexpr int kFineWaves = 4;      // wavefronts (tiles) per workgroup
 constexpr int kFineDefaultDW = 2;  // tile width in dwords when gol_config.lane_dwords is 0
+constexpr int kFineWideDW = 4;     // the width busy stretches switch to when lane_dwords is 0
+// Width switching (lane_dwords 0): the host reads a block's list length
+// every kFineChunk blocks, kFineLag - 1 chunks behind the GPU, and moves to
+// 128-cell tiles when it exceeds kFineWidePercent of the 64-cell tiles, back
+// when it falls below kFineNarrowPercent of the 128-cell tiles.
+constexpr int kFineChunk = 32;
+constexpr int kFineLag = 3;
+constexpr int kFineWidePercent = 12;
+constexpr int kFineNarrowPercent = 5;
 constexpr int kFineMaxSteps = 21;  // T <= tile_rows: a tile is at least as tall as T generations reach
 // Blocks of T <= 16 generations need at most 16 halo cells on each side of a
 // tile: one dword holds both (the ring form, DW + 1 dwords per lane); deeper
@@ -173,6 +182,9 @@ struct FineTrack {
     int* count_next;                // its sub-lists' lengths (0 when the block starts)
     int* count_clear;               // lengths of the list after next: reset to 0
     unsigned long long* wg_active;  // tiles advanced, per workgroup
+    unsigned long long* sample;     // or null: host-visible memory that receives, per sub-list j,
+                                    // sample_tag << 32 | the length of the sub-list this block advances
+    int sample_tag;                 // > 0
     int list_cap;                   // entries per sub-list: set by launch_fine_tiles (fine_list_cap)
 };
 

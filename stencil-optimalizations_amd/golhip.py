@@ -100,6 +100,7 @@ class GolInfo(ctypes.Structure):
         ("device_bytes", ctypes.c_uint64),
         ("native_layout", ctypes.c_uint32),
         ("workgroups", ctypes.c_uint32),
+        ("tile_width_switches", ctypes.c_uint32),
     ]
 
 

@@ -242,6 +242,36 @@ def test_split_calls_and_reupload():
         assert np.array_equal(g.download(), ref)
 
 
+def test_width_adapts_on_a_busy_grid():
+    # a dense soup lists far more than 12 % of the 64-cell tiles: the step
+    # moves to 128-cell tiles once the host has read a chunk's sample; exact
+    # throughout, also across calls and after a new upload (which starts on
+    # 64-cell tiles again)
+    x, y = 4096, 2048
+    w = golhip.fill_bernoulli(x, y, 23, 0.35)
+    ref1 = ol.bitpar_run(w, x, y, 2500)
+    ref2 = ol.bitpar_run(ref1, x, y, 3000)
+    with golhip.Grid(x, y, kernel=golhip.KERNEL_ACTIVE) as g:
+        g.upload(w)
+        note("active width switching", x, y)
+        g.step(2500)
+        assert np.array_equal(g.download(), ref1)
+        first = g.info()
+        assert first.tile_width_switches >= 1
+        g.step(3000)
+        assert np.array_equal(g.download(), ref2)
+        g.upload(w)
+        assert g.info().lane_dwords == 2
+        g.step(2500)
+        assert np.array_equal(g.download(), ref1)
+    # an explicit width never changes
+    with golhip.Grid(x, y, kernel=golhip.KERNEL_ACTIVE, lane_dwords=2) as g:
+        g.upload(w)
+        g.step(2500)
+        assert np.array_equal(g.download(), ref1)
+        assert g.info().tile_width_switches == 0
+
+
 def reference_workload_input(f):
     """The inputs of the reference's change-tracking workloads
     (cluster-experiments/generate-final-measurements.py:21, 27-30)."""
@@ -269,6 +299,9 @@ def test_reference_change_tracking_workloads(name):
     note("active reference workload", name)
     out, info = run(w, f["x"], f["y"], 10000)
     assert ol.sha256(out) == f["sha256"]
+    # the busy ones move to 128-cell tiles while their fronts collide
+    if "work" in name:
+        assert info.tile_width_switches >= 1
 
 
 def test_full_size_patches_20000():
@@ -278,4 +311,6 @@ def test_full_size_patches_20000():
     note("active 16384^2 patches 20000")
     out, info = run(w, 16384, 16384, 20000)
     assert ol.sha256(out) == f["sha256"]
+    # sparse throughout: it stays on 64-cell tiles
+    assert info.tile_width_switches == 0
     print("active fraction", info.active_tiles_total / (info.tiles * info.tracked_blocks))
