@@ -18,6 +18,13 @@ import os
 import sys
 import time
 
+# Hardware queues HIP gives this process (read once, when the runtime starts). The server runs six
+# streams (four lanes, uploads, downloads); with HIP's default of 4 some lanes share an in-order queue
+# with a copy stream or with each other, which streams share depending on the streams the process
+# created before. 4 (the default, also when set explicitly) becomes 8; any other setting is kept.
+if os.environ.get("GPU_MAX_HW_QUEUES", "4") == "4":
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "stencil-optimalizations_amd")
 sys.path.insert(0, PKG)
@@ -604,6 +611,7 @@ def main() -> int:
         "dtype_note": "bit logic on 32-bit lane words: each rows-64 word is processed as its two dwords, "
                       "same bytes, no width lost",
         "data": "synthetic",
+        "hip_hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),
         "config": {
             "workload": f"{X}x{Y} random (std::mt19937 seed {SEED}, p=0.5), {GENERATIONS} generations "
                         f"per step, dead boundary, rows-64 packing",
