@@ -272,6 +272,29 @@ def test_width_adapts_on_a_busy_grid():
         assert g.info().tile_width_switches == 0
 
 
+def test_width_returns_when_the_soup_settles():
+    # a 1024^2 soup at 0.35 lists over 12 % of the 64-cell tiles for its first
+    # few thousand generations and under 5 % of the 128-cell tiles by about
+    # generation 8000 (the listing rule stepped on the CPU: 0.49 of 64-cell
+    # tiles at 2048, 0.035 of 128-cell tiles at 8192, 0.004 from 12288 on):
+    # one step moves to 128-cell tiles and back, exact; in pieces too
+    x, y, gens = 1024, 1024, 12000
+    w = golhip.fill_bernoulli(x, y, 23, 0.35)
+    ref = ol.bitpar_run(w, x, y, gens)
+    with golhip.Grid(x, y, kernel=golhip.KERNEL_ACTIVE) as g:
+        g.upload(w)
+        note("active width back", x, y, gens)
+        g.step(gens)
+        assert np.array_equal(g.download(), ref)
+        info = g.info()
+        assert info.tile_width_switches >= 2 and info.lane_dwords == 2
+        g.upload(w)
+        for _ in range(4):
+            g.step(gens // 4)
+        assert np.array_equal(g.download(), ref)
+        assert g.info().lane_dwords == 2
+
+
 def reference_workload_input(f):
     """The inputs of the reference's change-tracking workloads
     (cluster-experiments/generate-final-measurements.py:21, 27-30)."""
