@@ -106,7 +106,7 @@ constexpr int kFineWideDW = 4;     // the width busy stretches switch to when la
 // when it falls below kFineNarrowPercent of the 128-cell tiles.
 constexpr int kFineChunk = 32;
 constexpr int kFineLag = 3;
-constexpr int kFineWidePercent = 12;
+constexpr int kFineWidePercent = 7;
 constexpr int kFineNarrowPercent = 5;
 constexpr int kFineMaxSteps = 21;  // T <= tile_rows: a tile is at least as tall as T generations reach
 // Blocks of T <= 16 generations need at most 16 halo cells on each side of a

@@ -10,6 +10,9 @@
                                   (the bench's mostly-stable configuration)
   python3 profile_run.py spacefiller  16384^2 empty with spacefiller[8192, 8192], 20000 generations,
                                   change tracking (the bench's spacefiller configuration)
+  python3 profile_run.py spacefillers30  16384^2 with 30 spacefillers (6 x 5 positions, the
+                                  reference's 66 %-busy change-tracking workload), 10000 generations,
+                                  change tracking with the tile width left to the library
   python3 profile_run.py batch    the bench's batched workload: 4096 uneven grids (sides 64..2048,
                                   100..5000 generations), built once, then two graph replays
   python3 profile_run.py batch_queue  the same batch as persistent queue launches (one per kernel
@@ -80,6 +83,16 @@ def main(what: str) -> None:
                 print(f"{what}: 20000 generations, {info.launches} launches, {ms:.2f} ms GPU, "
                       f"{info.active_tiles_total} active tile-blocks of {info.tiles * info.tracked_blocks}",
                       flush=True)
+    elif what == "spacefillers30":
+        expr = "; ".join(f"spacefiller[{16384 // 7 * i}, {16384 // 6 * j}]" for i in range(1, 7) for j in range(1, 6))
+        w = golhip.fill_lexicon(16384, 16384, expr)
+        with golhip.Grid(16384, 16384, kernel=golhip.KERNEL_ACTIVE, device=0) as g:
+            g.upload(w)
+            ms = g.step_timed(10000)
+            info = g.info()
+        print(f"{what}: 10000 generations, {info.launches} launches, {ms:.2f} ms GPU, "
+              f"{info.active_tiles_total} active tile-blocks, {info.tile_width_switches} tile-width switches",
+              flush=True)
     elif what in ("batch", "batch_queue"):
         specs = workloads.batched_uneven(4096)
         inputs = workloads.make_inputs(specs)

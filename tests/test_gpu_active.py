@@ -243,7 +243,7 @@ def test_split_calls_and_reupload():
 
 
 def test_width_adapts_on_a_busy_grid():
-    # a dense soup lists far more than 12 % of the 64-cell tiles: the step
+    # a dense soup lists far more than 7 % of the 64-cell tiles: the step
     # moves to 128-cell tiles once the host has read a chunk's sample; exact
     # throughout, also across calls and after a new upload (which starts on
     # 64-cell tiles again)
@@ -273,7 +273,7 @@ def test_width_adapts_on_a_busy_grid():
 
 
 def test_width_returns_when_the_soup_settles():
-    # a 1024^2 soup at 0.35 lists over 12 % of the 64-cell tiles for its first
+    # a 1024^2 soup at 0.35 lists over 7 % of the 64-cell tiles for its first
     # few thousand generations and under 5 % of the 128-cell tiles by about
     # generation 8000 (the listing rule stepped on the CPU: 0.49 of 64-cell
     # tiles at 2048, 0.035 of 128-cell tiles at 8192, 0.004 from 12288 on):
