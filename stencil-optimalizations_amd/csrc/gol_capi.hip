@@ -891,10 +891,14 @@ int gol_get_info(gol_handle* h, gol_info* info) {
         info->tracked_blocks = h->blocks;
         info->tile_width_switches = static_cast<uint32_t>(h->fine_switches);
         info->workgroups = static_cast<uint32_t>(per_wg.size());
+        // as allocated: each buffer for the larger of the two geometries
         const FineGeo& g0 = h->fine_geo[0];
-        info->device_bytes += static_cast<uint64_t>(g0.tiles_x) * g0.tiles_y * 3 * sizeof(uint32_t) +
-                              3ull * kFineLists * fine_list_cap(g0) * sizeof(int) +
-                              2ull * fine_words(g0) * sizeof(uint32_t) +
+        const FineGeo& g1 = h->fine_geo[1];
+        const uint64_t nt = std::max<uint64_t>(static_cast<uint64_t>(g0.tiles_x) * g0.tiles_y,
+                                               static_cast<uint64_t>(g1.tiles_x) * g1.tiles_y);
+        info->device_bytes += nt * 3 * sizeof(uint32_t) +
+                              3ull * kFineLists * std::max(fine_list_cap(g0), fine_list_cap(g1)) * sizeof(int) +
+                              2ull * std::max(fine_words(g0), fine_words(g1)) * sizeof(uint32_t) +
                               3ull * kFineCounts * sizeof(int) + per_wg.size() * sizeof(unsigned long long);
     }
     return GOL_OK;
