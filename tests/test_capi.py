@@ -165,3 +165,44 @@ def test_noalloc_errors_before_device(args, message):
         golhip.step_device_noalloc(*args)
     assert e.value.code == golhip.GOL_ERR_INVALID_ARGUMENT
     assert message in str(e.value)
+
+
+# The ctypes mirrors of the header's structs: every field at the offset the C
+# compiler gives it, every struct the same size (a field added to one side
+# only, or in another order, fails here).
+ABI_STRUCTS = [
+    ("gol_config", golhip.GolConfig), ("gol_info", golhip.GolInfo),
+    ("gol_batch_config", golhip.GolBatchConfig), ("gol_batch_info", golhip.GolBatchInfo),
+    ("gol_tune_config", golhip.GolTuneConfig), ("gol_tune_entry", golhip.GolTuneEntry),
+    ("gol_server_config", golhip.GolServerConfig), ("gol_job_stats", golhip.GolJobStats),
+    ("gol_server_stats", golhip.GolServerStats),
+]
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    import subprocess
+
+    lines = ["#include <stdio.h>", "#include \"golhip.h\"", "int main(void) {"]
+    for cname, cls in ABI_STRUCTS:
+        lines.append(f'    printf("{cname} %zu\\n", sizeof({cname}));')
+        for field in cls._fields_:
+            lines.append(f'    printf("{cname}.{field[0]} %zu\\n", offsetof({cname}, {field[0]}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-I", os.path.dirname(HEADER), "-o", str(exe), str(src)],
+                   check=True)
+    c_side = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                                  text=True).stdout.splitlines())
+    for cname, cls in ABI_STRUCTS:
+        assert int(c_side[cname]) == ctypes.sizeof(cls), cname
+        for field in cls._fields_:
+            assert int(c_side[f"{cname}.{field[0]}"]) == getattr(cls, field[0]).offset, f"{cname}.{field[0]}"
+    # and the mirrors name the header's fields, in its order
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    for cname, cls in ABI_STRUCTS:
+        body = re.search(r"typedef struct " + cname + r" \{(.*?)\} " + cname + ";", text, flags=re.S).group(1)
+        names = [re.sub(r"\[.*?\]", "", part).split()[-1]
+                 for decl in body.split(";") if decl.strip() for part in decl.split(",")]
+        assert names == [f[0] for f in cls._fields_], cname
